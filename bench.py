@@ -1,0 +1,295 @@
+"""Benchmark: FP64 adaptive DP5(4) ray tracing on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hd|small]
+
+One bench step traces the whole 1920x1080 sweep -- spins a in {0, 0.5, 0.9,
+0.998} x inclinations {17, 80} deg, camera at r = 50 M, horizontal FOV 25 deg,
+rtol 1e-8 / atol 1e-10, capture at r_+ + 0.01 M, escape at 100 M, max_steps
+20000 -- one plain-tracer launch per frame (8 launches, 16.6 M rays).  Rays
+are generated on the device before timing (camera directions on the host,
+null momenta and the SoA layout on the device), so the timed region is the
+tracer kernels only, with inputs resident in HBM.
+
+Prints one JSON line.  value = accepted ray-steps per second over the sweep.
+roofline: algorithmic FP64 FLOP of the tracer kernel (1321 FLOP per DP5
+attempt: 6 RHS x 146 + 406 stage + 36 error-norm + 3 controller) per launch,
+over the kernel's mean launch time measured with HIP events on its stream,
+against the FP64 vector peak (78.6 TFLOP/s).  cpu_baseline: the NumPy oracle
+(a restatement of the reference tracer) on a seeded subsample of every frame,
+on this host's cores, timed before the GPU is touched.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+FLOP_PER_ATTEMPT = 1321
+FP64_VECTOR_PEAK_TFLOPS = 78.6
+METRIC = "ray-steps/s FP64 @rtol 1e-8, % FP64 vector peak, adjoint/forward time"
+
+
+def frames(workload: str):
+    from blackhorizon_amd import workloads
+
+    if workload == "hd":
+        return workloads.hd_frames()
+    if workload == "hd09":
+        return [workloads.hd_frame(0.9, 17.0), workloads.hd_frame(0.9, 80.0)]
+    if workload == "small":
+        return [workloads.small_frame()]
+    raise SystemExit(f"unknown workload {workload}")
+
+
+def camera_arrays(job):
+    from blackhorizon_amd.camera import PinholeCamera
+
+    cam = PinholeCamera.from_orbit(job.distance, job.inclination_degrees,
+                                   azimuth_degrees=job.azimuth_degrees,
+                                   fov_degrees=job.fov_degrees, width=job.width,
+                                   height=job.height)
+    return cam.ray_origins(), cam.ray_directions()
+
+
+# --------------------------------------------------------------------------
+# CPU baseline (NumPy oracle), run before any GPU work
+# --------------------------------------------------------------------------
+
+def _cpu_shard(args):
+    job_dict, pos, dirs = args
+    from blackhorizon_amd.workloads import CameraJob
+    from oracle import bh_oracle as orc
+
+    job = CameraJob(**job_dict)
+    st = orc.Kerr(job.mass, job.spin)
+    s0 = orc.build_state(pos, orc.null_momentum_from_velocity(st, pos, dirs, "past"))
+    t0 = time.perf_counter()
+    res, att = orc.trace_rays(st, s0, job.escape_radius, max_steps=job.max_steps,
+                              rtol=job.rtol, atol=job.atol, initial_step=job.initial_step,
+                              max_step=job.max_step, min_step=job.min_step,
+                              capture_margin=job.capture_margin, return_attempts=True)
+    return time.perf_counter() - t0, int(res.steps.sum()), int(att.sum())
+
+
+def cpu_baseline(jobs, per_frame: int, seed: int = 7, processes: int = 1):
+    shards = []
+    for f, job in enumerate(jobs):
+        pos, dirs = camera_arrays(job)
+        rng = np.random.default_rng(seed + f)
+        idx = np.sort(rng.choice(job.n_rays, size=min(per_frame, job.n_rays), replace=False))
+        for p in range(processes):
+            sl = idx[p::processes]
+            shards.append((job.as_dict(), pos[sl], dirs[sl]))
+    t0 = time.perf_counter()
+    if processes == 1:
+        out = [_cpu_shard(s) for s in shards]
+    else:
+        import multiprocessing as mp
+
+        with mp.get_context("fork").Pool(processes) as pool:
+            out = pool.map(_cpu_shard, shards)
+    wall = time.perf_counter() - t0
+    steps = sum(o[1] for o in out)
+    attempts = sum(o[2] for o in out)
+    return dict(value=steps / wall, unit="ray-steps/s", cores=processes, kind="port",
+                sample=f"{min(per_frame, jobs[0].n_rays)} seeded rays of each of {len(jobs)} "
+                       f"frames ({len(jobs) * min(per_frame, jobs[0].n_rays)} rays), NumPy "
+                       f"oracle trace_rays, {processes} process(es)",
+                wall_s=round(wall, 3), ray_steps=steps, attempts=attempts,
+                ns_per_ray_step=wall / max(steps, 1) * 1e9)
+
+
+# --------------------------------------------------------------------------
+# GPU
+# --------------------------------------------------------------------------
+
+def prepare_gpu(jobs, mode):
+    import torch
+
+    from blackhorizon_amd import _lib, geodesics, kerr, tracer
+
+    lib = _lib.require_device()
+    dev = torch.device("cuda", 0)
+    stream = _lib.stream_handle()
+    prepared = []
+    for job in jobs:
+        st = kerr.KerrSpacetime(job.mass, job.spin)
+        pos, dirs = camera_arrays(job)
+        pos_t = torch.from_numpy(pos).to(dev)
+        mom = geodesics.null_momentum_from_velocity(st, pos_t, torch.from_numpy(dirs).to(dev),
+                                                    "past")
+        state = geodesics.build_state(pos_t, mom)
+        n = job.n_rays
+        soa_in = torch.empty((8, n), dtype=torch.float64, device=dev)
+        _lib.check(lib.bh_aos_to_soa_f64(_lib.ptr(state), _lib.ptr(soa_in), n, n, stream), "soa")
+        params = tracer.make_params(st, job.escape_radius, job.max_steps, job.rtol, job.atol,
+                                    job.initial_step, job.max_step, job.min_step,
+                                    job.capture_margin, mode)
+        bufs = dict(soa_in=soa_in, soa_out=torch.empty_like(soa_in),
+                    status=torch.empty(n, dtype=torch.int32, device=dev),
+                    steps=torch.empty(n, dtype=torch.int64, device=dev),
+                    attempts=torch.empty(n, dtype=torch.int64, device=dev),
+                    iters=torch.zeros(1, dtype=torch.int64, device=dev))
+        prepared.append((job, params, bufs))
+    torch.cuda.synchronize()
+    return prepared
+
+
+def launch(lib, params, tune, b, stream):
+    from blackhorizon_amd import _lib
+
+    n = b["soa_in"].shape[1]
+    _lib.check(lib.bh_trace_plain_f64(params, tune, _lib.ptr(b["soa_in"]), n,
+                                      _lib.ptr(b["soa_out"]), n, _lib.ptr(b["status"]),
+                                      _lib.ptr(b["steps"]), _lib.ptr(b["attempts"]),
+                                      _lib.ptr(b["iters"]), n, stream), "trace")
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="hd", choices=["hd", "hd09", "small"])
+    ap.add_argument("--mode", default="fast", choices=["fast", "precise"])
+    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--rays-per-lane", type=int, default=1)
+    ap.add_argument("--refill-min", type=int, default=1)
+    ap.add_argument("--cpu-rays-per-frame", type=int, default=2048)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args(argv)
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build targets one GPU: the tracer shards rays with no "
+                                   "exchange, but only the single-GPU path is implemented and "
+                                   "measured; run with --gpus 1"}))
+        return 2
+
+    jobs = frames(args.workload)
+    cpu = cpu_par = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(jobs, args.cpu_rays_per_frame, processes=1)
+        p = min(len(os.sched_getaffinity(0)), 16)
+        if p > 1:
+            cpu_par = cpu_baseline(jobs, args.cpu_rays_per_frame, processes=p)
+        print(f"[bench] cpu baseline done: {cpu['ns_per_ray_step']:.0f} ns/ray-step (1 core)",
+              file=sys.stderr, flush=True)
+
+    import torch
+
+    from blackhorizon_amd import _lib, tracer
+
+    lib = _lib.require_device()
+    prepared = prepare_gpu(jobs, args.mode)
+    tune = tracer.Tune(args.block, args.rays_per_lane, args.refill_min).as_c()
+    stream = _lib.stream_handle()
+    cur = torch.cuda.current_stream()
+
+    for w in range(args.warmup):
+        for job, params, b in prepared:
+            launch(lib, params, tune, b, stream)
+        torch.cuda.synchronize()
+        print(f"[bench] warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+
+    events = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        for job, params, b in prepared:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(cur)
+            launch(lib, params, tune, b, stream)
+            e1.record(cur)
+            events.append((e0, e1))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+
+    kern_ms = [a.elapsed_time(b) for a, b in events]
+    per_frame = []
+    steps_sweep = attempts_sweep = 0
+    for i, (job, params, b) in enumerate(prepared):
+        s = int(b["steps"].sum().item())
+        a = int(b["attempts"].sum().item())
+        st = torch.bincount(b["status"].long(), minlength=7).tolist()
+        ms = [kern_ms[k * len(prepared) + i] for k in range(args.steps)]
+        per_frame.append(dict(spin=job.spin, inclination=job.inclination_degrees,
+                              rays=job.n_rays, ray_steps=s, attempts=a,
+                              iterations=int(b["iters"].item()),
+                              kernel_ms=round(float(np.mean(ms)), 3),
+                              ns_per_ray_step=float(np.mean(ms)) * 1e6 / max(s, 1),
+                              pct_fp64_peak=100.0 * a * FLOP_PER_ATTEMPT
+                              / (float(np.mean(ms)) * 1e-3) / (FP64_VECTOR_PEAK_TFLOPS * 1e12),
+                              status_counts=st[:7]))
+        steps_sweep += s
+        attempts_sweep += a
+    total_kernel_s = sum(kern_ms) * 1e-3
+    value = steps_sweep * args.steps / wall
+    flop = attempts_sweep * args.steps * FLOP_PER_ATTEMPT
+    achieved_tflops = flop / total_kernel_s / 1e12
+    props = torch.cuda.get_device_properties(0)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "ray-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": {"hd": "1920x1080 sweep, 8 frames: a in {0,0.5,0.9,0.998} x inc "
+                               "{17,80} deg, r_obs 50 M, FOV 25 deg, rtol 1e-8, atol 1e-10",
+                         "hd09": "1920x1080, a = 0.9, inc {17,80} deg, r_obs 50 M, FOV 25 deg, "
+                                 "rtol 1e-8, atol 1e-10",
+                         "small": "160x120, a = 0.9, r_obs 30 M, inc 60 deg, FOV 30 deg, "
+                                  "rtol 1e-8, atol 1e-10"}[args.workload],
+            "rays_per_step": int(sum(j.n_rays for j in jobs)),
+            "launches_per_step": len(jobs),
+            "mode": args.mode,
+            "tune": {"block": args.block, "rays_per_lane": args.rays_per_lane,
+                     "refill_min": args.refill_min},
+        },
+        "ns_per_ray_step": wall / (steps_sweep * args.steps) * 1e9,
+        "attempts_per_s": attempts_sweep * args.steps / wall,
+        "pct_fp64_vector_peak": 100.0 * achieved_tflops / FP64_VECTOR_PEAK_TFLOPS,
+        "adjoint_over_forward": None,
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved_tflops,
+            "peak": FP64_VECTOR_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tflops / FP64_VECTOR_PEAK_TFLOPS,
+            "traffic": None,
+            "kernel": "trace_plain_kernel (fast)" if args.mode == "fast"
+                      else "trace_plain_kernel (precise)",
+            "flop_per_launch": flop / len(events),
+            "mean_launch_ms": float(np.mean(kern_ms)),
+        },
+        "cpu_baseline": None if cpu is None else {
+            k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+        "cpu_baseline_detail": cpu,
+        "cpu_baseline_parallel": cpu_par,
+        "gpu": {"name": props.name, "cus": props.multi_processor_count},
+        "per_frame": per_frame,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,37 @@
+"""Device-buffer plumbing shared by the API modules: move inputs to the
+HIP device as contiguous float64 tensors and return results in the caller's
+array module (NumPy in -> NumPy out, tensor in -> tensor out)."""
+
+from __future__ import annotations
+
+import numpy
+import torch
+
+from . import _lib
+
+
+def device() -> torch.device:
+    _lib.require_device()
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def as_device_f64(a, shape_tail: tuple | None = None, name: str = "array") -> torch.Tensor:
+    """Contiguous float64 tensor on the current HIP device."""
+    dev = device()
+    if isinstance(a, torch.Tensor):
+        t = a.detach().to(device=dev, dtype=torch.float64)
+    else:
+        t = torch.from_numpy(numpy.ascontiguousarray(numpy.asarray(a, dtype=numpy.float64))).to(dev)
+    t = t.contiguous()
+    if shape_tail is not None:
+        if t.dim() != 1 + len(shape_tail) or tuple(t.shape[1:]) != tuple(shape_tail):
+            raise ValueError(f"{name} must have shape (n, {', '.join(map(str, shape_tail))}), "
+                             f"got {tuple(t.shape)}")
+    return t
+
+
+def like_input(t: torch.Tensor, ref):
+    """Return ``t`` as the caller's array type."""
+    if isinstance(ref, torch.Tensor):
+        return t
+    return t.cpu().numpy()

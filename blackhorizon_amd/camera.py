@@ -1,0 +1,82 @@
+"""Pinhole camera (reference camera.py:24-120).
+
+Host-side, like the reference: it maps pixels to world directions with
+NumPy; photon momenta are then built on the device by
+``geodesics.null_momentum_from_velocity``.  Horizontal FOV; pixel (row, col)
+is ray ``row * width + col`` from the top-left corner.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy
+
+
+def _normalize(v: numpy.ndarray) -> numpy.ndarray:
+    norm = numpy.linalg.norm(v)
+    if norm == 0.0:
+        raise ValueError("zero-length vector cannot be normalized")
+    return v / norm
+
+
+@dataclass(frozen=True)
+class PinholeCamera:
+    position: tuple
+    target: tuple = (0.0, 0.0, 0.0)
+    up_hint: tuple = (0.0, 0.0, 1.0)
+    fov_degrees: float = 20.0
+    width: int = 320
+    height: int = 240
+
+    forward: numpy.ndarray = field(init=False, repr=False)
+    right: numpy.ndarray = field(init=False, repr=False)
+    up: numpy.ndarray = field(init=False, repr=False)
+
+    def __post_init__(self) -> None:
+        # camera.py:48-66
+        if self.width <= 0 or self.height <= 0:
+            raise ValueError("image dimensions must be positive")
+        if not 0.0 < self.fov_degrees < 180.0:
+            raise ValueError("fov_degrees must be in (0, 180)")
+        forward = _normalize(numpy.asarray(self.target, dtype=numpy.float64)
+                             - numpy.asarray(self.position, dtype=numpy.float64))
+        right = numpy.cross(forward, numpy.asarray(self.up_hint, dtype=numpy.float64))
+        if numpy.linalg.norm(right) < 1e-12:
+            right = numpy.cross(forward, numpy.array([0.0, 1.0, 0.0]))
+        right = _normalize(right)
+        up = numpy.cross(right, forward)
+        object.__setattr__(self, "forward", forward)
+        object.__setattr__(self, "right", right)
+        object.__setattr__(self, "up", up)
+
+    @classmethod
+    def from_orbit(cls, distance: float, inclination_degrees: float,
+                   azimuth_degrees: float = 0.0, **kwargs) -> "PinholeCamera":
+        """Camera on a sphere of radius ``distance`` (camera.py:68-94)."""
+        if distance <= 0.0:
+            raise ValueError("distance must be positive")
+        inc = math.radians(inclination_degrees)
+        azi = math.radians(azimuth_degrees)
+        position = (distance * math.sin(inc) * math.cos(azi),
+                    distance * math.sin(inc) * math.sin(azi),
+                    distance * math.cos(inc))
+        return cls(position=position, **kwargs)
+
+    def ray_directions(self) -> numpy.ndarray:
+        """Unit directions, (height * width, 3), row-major (camera.py:96-113)."""
+        half_w = math.tan(math.radians(self.fov_degrees) / 2.0)
+        half_h = half_w * self.height / self.width
+        cols = (numpy.arange(self.width) + 0.5) / self.width * 2.0 - 1.0
+        rows = 1.0 - (numpy.arange(self.height) + 0.5) / self.height * 2.0
+        u, v = numpy.meshgrid(cols * half_w, rows * half_h)
+        d = (self.forward[None, None, :] + u[..., None] * self.right[None, None, :]
+             + v[..., None] * self.up[None, None, :])
+        d /= numpy.linalg.norm(d, axis=-1, keepdims=True)
+        return d.reshape(-1, 3)
+
+    def ray_origins(self) -> numpy.ndarray:
+        """Camera position per pixel, (height * width, 3) (camera.py:115-120)."""
+        o = numpy.asarray(self.position, dtype=numpy.float64)
+        return numpy.broadcast_to(o, (self.height * self.width, 3)).copy()

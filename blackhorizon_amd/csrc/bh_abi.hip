@@ -1,0 +1,247 @@
+// C ABI of the tracer (declared in include/bh_trace.h): argument checks,
+// arithmetic-mode dispatch and the small layout/fill kernels.
+#include <stdio.h>
+#include <string.h>
+
+#include "bh_internal.h"
+
+namespace bh {
+
+static thread_local char g_err[512] = "";
+
+static int32_t fail(int32_t code, const char *fmt, const char *detail = nullptr) {
+  if (detail) snprintf(g_err, sizeof(g_err), fmt, detail);
+  else snprintf(g_err, sizeof(g_err), "%s", fmt);
+  return code;
+}
+
+static int32_t hip_status(hipError_t e, const char *where) {
+  if (e == hipSuccess) return BH_OK;
+  snprintf(g_err, sizeof(g_err), "%s: HIP error %d (%s)", where, (int)e, hipGetErrorString(e));
+  return BH_EHIP;
+}
+
+__global__ void aos_to_soa_kernel(const double *aos, double *soa, int64_t ld, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) soa[c * ld + i] = aos[8 * i + c];
+}
+
+__global__ void soa_to_aos_kernel(const double *soa, int64_t ld, double *aos, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) aos[8 * i + c] = soa[c * ld + i];
+}
+
+__global__ void zero_u64_kernel(unsigned long long *p) { *p = 0ull; }
+__global__ void zero_i32_kernel(int32_t *p) { *p = 0; }
+
+hipError_t launch_aos_to_soa(const double *aos, double *soa, int64_t ld, int64_t n,
+                             hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(aos_to_soa_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, aos,
+                     soa, ld, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_soa_to_aos(const double *soa, int64_t ld, double *aos, int64_t n,
+                             hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(soa_to_aos_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, soa,
+                     ld, aos, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_u64(unsigned long long *p, hipStream_t s) {
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, s, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_i32(int32_t *p, hipStream_t s) {
+  hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(1), 0, s, p);
+  return hipGetLastError();
+}
+
+// Spacetime checks of KerrSpacetime.__init__ (kerr.py:68-71).
+static int32_t check_spacetime(double mass, double spin) {
+  if (!(mass > 0.0)) return fail(BH_EINVAL, "mass must be positive");
+  if (!(fabs(spin) <= mass)) return fail(BH_EINVAL, "Kerr bound violated: |spin| must be <= mass");
+  return BH_OK;
+}
+
+static int32_t check_params(const bh_trace_params *p) {
+  if (!p) return fail(BH_EINVAL, "params is NULL");
+  int32_t rc = check_spacetime(p->mass, p->spin);
+  if (rc) return rc;
+  if (p->mode != BH_MODE_FAST && p->mode != BH_MODE_PRECISE)
+    return fail(BH_EINVAL, "mode must be BH_MODE_FAST or BH_MODE_PRECISE");
+  if (p->max_steps > (int64_t)1 << 60) return fail(BH_EINVAL, "max_steps too large");
+  return BH_OK;
+}
+
+static JobConst spacetime_const(double mass, double spin) {
+  bh_trace_params p;
+  memset(&p, 0, sizeof(p));
+  p.mass = mass;
+  p.spin = spin;
+  return make_job_const(p);
+}
+
+}  // namespace bh
+
+using namespace bh;
+
+extern "C" {
+
+int32_t bh_abi_version(void) { return BH_ABI_VERSION; }
+
+const char *bh_last_error(void) { return g_err; }
+
+int32_t bh_count_outside_f64(const bh_trace_params *p, const double *state_soa, int64_t ld,
+                             int64_t n, int32_t *count_out, void *stream) {
+  int32_t rc = check_params(p);
+  if (rc) return rc;
+  if (n < 0 || ld < n) return fail(BH_EINVAL, "bad n / ld");
+  if (!count_out || (n > 0 && !state_soa)) return fail(BH_EINVAL, "NULL buffer");
+  hipStream_t s = (hipStream_t)stream;
+  JobConst c = make_job_const(*p);
+  hipError_t e = launch_zero_i32(count_out, s);
+  if (e == hipSuccess)
+    e = p->mode == BH_MODE_PRECISE ? precise::launch_count_outside(c, state_soa, ld, n, count_out, s)
+                                   : fast::launch_count_outside(c, state_soa, ld, n, count_out, s);
+  return hip_status(e, "bh_count_outside_f64");
+}
+
+int32_t bh_trace_plain_f64(const bh_trace_params *p, const bh_tune_params *tune,
+                           const double *state_in_soa, int64_t ld_in, double *state_out_soa,
+                           int64_t ld_out, int32_t *status, int64_t *steps, int64_t *attempts,
+                           uint64_t *iterations, int64_t n, void *stream) {
+  int32_t rc = check_params(p);
+  if (rc) return rc;
+  if (n < 0 || ld_in < n || ld_out < n) return fail(BH_EINVAL, "bad n / ld");
+  if (n > 0 && (!state_in_soa || !state_out_soa || !status || !steps || !attempts))
+    return fail(BH_EINVAL, "NULL buffer");
+  LaunchShape shape{256, 1, 1};
+  if (tune) {
+    if (tune->block_size) shape.block = tune->block_size;
+    if (tune->rays_per_lane) shape.rays_per_lane = tune->rays_per_lane;
+    if (tune->refill_min) shape.refill_min = tune->refill_min;
+  }
+  if (shape.block <= 0 || shape.block > 256 || shape.block % 64)
+    return fail(BH_EINVAL, "block_size must be a multiple of 64 in [64, 256]");
+  if (shape.rays_per_lane < 1) return fail(BH_EINVAL, "rays_per_lane must be >= 1");
+  if (shape.refill_min < 1 || shape.refill_min > 64)
+    return fail(BH_EINVAL, "refill_min must be in [1, 64]");
+  hipStream_t s = (hipStream_t)stream;
+  JobConst c = make_job_const(*p);
+  TraceIO io{state_in_soa, ld_in, state_out_soa, ld_out, status, steps, attempts,
+             (unsigned long long *)iterations, n};
+  hipError_t e = hipSuccess;
+  if (iterations) e = launch_zero_u64((unsigned long long *)iterations, s);
+  if (e == hipSuccess)
+    e = p->mode == BH_MODE_PRECISE ? precise::launch_trace_plain(c, io, shape, s)
+                                   : fast::launch_trace_plain(c, io, shape, s);
+  return hip_status(e, "bh_trace_plain_f64");
+}
+
+int32_t bh_aos_to_soa_f64(const double *aos, double *soa, int64_t ld, int64_t n, void *stream) {
+  if (n < 0 || ld < n) return fail(BH_EINVAL, "bad n / ld");
+  if (n > 0 && (!aos || !soa)) return fail(BH_EINVAL, "NULL buffer");
+  return hip_status(launch_aos_to_soa(aos, soa, ld, n, (hipStream_t)stream), "bh_aos_to_soa_f64");
+}
+
+int32_t bh_soa_to_aos_f64(const double *soa, int64_t ld, double *aos, int64_t n, void *stream) {
+  if (n < 0 || ld < n) return fail(BH_EINVAL, "bad n / ld");
+  if (n > 0 && (!aos || !soa)) return fail(BH_EINVAL, "NULL buffer");
+  return hip_status(launch_soa_to_aos(soa, ld, aos, n, (hipStream_t)stream), "bh_soa_to_aos_f64");
+}
+
+int32_t bh_geometry_f64(double mass, double spin, int32_t mode, const double *xyz, int64_t n,
+                        double *radius, double *h, double *l, double *grad_h, double *grad_l,
+                        void *stream) {
+  int32_t rc = check_spacetime(mass, spin);
+  if (rc) return rc;
+  if (n < 0) return fail(BH_EINVAL, "bad n");
+  if (n > 0 && !xyz) return fail(BH_EINVAL, "NULL buffer");
+  JobConst c = spacetime_const(mass, spin);
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = mode == BH_MODE_PRECISE
+                     ? precise::launch_geometry(c, xyz, n, radius, h, l, grad_h, grad_l, s)
+                     : fast::launch_geometry(c, xyz, n, radius, h, l, grad_h, grad_l, s);
+  return hip_status(e, "bh_geometry_f64");
+}
+
+int32_t bh_geodesic_rhs_f64(double mass, double spin, int32_t mode, const double *state,
+                            int64_t n, double *deriv, void *stream) {
+  int32_t rc = check_spacetime(mass, spin);
+  if (rc) return rc;
+  if (n < 0) return fail(BH_EINVAL, "bad n");
+  if (n > 0 && (!state || !deriv)) return fail(BH_EINVAL, "NULL buffer");
+  JobConst c = spacetime_const(mass, spin);
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = mode == BH_MODE_PRECISE ? precise::launch_rhs(c, state, n, deriv, s)
+                                         : fast::launch_rhs(c, state, n, deriv, s);
+  return hip_status(e, "bh_geodesic_rhs_f64");
+}
+
+int32_t bh_dp5_step_f64(double mass, double spin, int32_t mode, double rtol, double atol,
+                        const double *state, const double *h, int64_t n, double *new_state,
+                        double *error, double *ratio, double *factor, void *stream) {
+  int32_t rc = check_spacetime(mass, spin);
+  if (rc) return rc;
+  if (n < 0) return fail(BH_EINVAL, "bad n");
+  if (n > 0 && (!state || !h || !new_state)) return fail(BH_EINVAL, "NULL buffer");
+  JobConst c = spacetime_const(mass, spin);
+  c.rtol = rtol;
+  c.atol = atol;
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e =
+      mode == BH_MODE_PRECISE
+          ? precise::launch_dp5_step(c, state, h, n, new_state, error, ratio, factor, s)
+          : fast::launch_dp5_step(c, state, h, n, new_state, error, ratio, factor, s);
+  return hip_status(e, "bh_dp5_step_f64");
+}
+
+int32_t bh_null_momentum_f64(double mass, double spin, const double *positions,
+                             const double *directions, int64_t n, int32_t past,
+                             int32_t normalize, double *momenta, void *stream) {
+  int32_t rc = check_spacetime(mass, spin);
+  if (rc) return rc;
+  if (n < 0) return fail(BH_EINVAL, "bad n");
+  if (n > 0 && (!positions || !directions || !momenta)) return fail(BH_EINVAL, "NULL buffer");
+  JobConst c = spacetime_const(mass, spin);
+  return hip_status(precise::launch_null_momentum(c, positions, directions, n, past ? 1 : 0,
+                                                  normalize ? 1 : 0, momenta, (hipStream_t)stream),
+                    "bh_null_momentum_f64");
+}
+
+int32_t bh_timelike_momentum_f64(double mass, double spin, const double *positions,
+                                 const double *velocities, int64_t n, double *momenta,
+                                 int32_t *bad_count, void *stream) {
+  int32_t rc = check_spacetime(mass, spin);
+  if (rc) return rc;
+  if (n < 0) return fail(BH_EINVAL, "bad n");
+  if (n > 0 && (!positions || !velocities || !momenta)) return fail(BH_EINVAL, "NULL buffer");
+  JobConst c = spacetime_const(mass, spin);
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = bad_count ? launch_zero_i32(bad_count, s) : hipSuccess;
+  if (e == hipSuccess)
+    e = precise::launch_timelike_momentum(c, positions, velocities, n, momenta, bad_count, s);
+  return hip_status(e, "bh_timelike_momentum_f64");
+}
+
+int32_t bh_diagnostics_f64(double mass, double spin, const double *state, int64_t n, double *ham,
+                           double *energy, double *l_z, double *velocity, void *stream) {
+  int32_t rc = check_spacetime(mass, spin);
+  if (rc) return rc;
+  if (n < 0) return fail(BH_EINVAL, "bad n");
+  if (n > 0 && !state) return fail(BH_EINVAL, "NULL buffer");
+  JobConst c = spacetime_const(mass, spin);
+  return hip_status(
+      precise::launch_diagnostics(c, state, n, ham, energy, l_z, velocity, (hipStream_t)stream),
+      "bh_diagnostics_f64");
+}
+
+}  // extern "C"

@@ -1,0 +1,103 @@
+// Internal declarations shared by the two arithmetic builds of the kernels
+// (bh_precise.hip, bh_fast.hip) and the C-ABI layer (bh_abi.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/bh_trace.h"
+
+namespace bh {
+
+// Kernel-side copy of one job's controller settings, with the spin-derived
+// constants the geometry uses.  Each derived constant is the same double the
+// reference forms from Python floats (kerr.py:134,138,165,166), so the
+// per-ray arithmetic sees identical operands.
+struct JobConst {
+  double m, a, a2;
+  double four_a2;    // 4.0 * a2         (kerr.py:138)
+  double three_a2;   // 3.0 * a2         (kerr.py:165)
+  double dhz_c;      // -2.0 * m * a2    (kerr.py:166)
+  double escape_radius, capture_radius;
+  double rtol, atol;
+  double initial_step, max_step, min_step;
+  int64_t max_steps;
+  int64_t max_attempts;  // 4 * max_steps  (tracer.py:101)
+};
+
+inline JobConst make_job_const(const bh_trace_params &p) {
+  JobConst c;
+  c.m = p.mass;
+  c.a = p.spin;
+  c.a2 = p.spin * p.spin;
+  c.four_a2 = 4.0 * c.a2;
+  c.three_a2 = 3.0 * c.a2;
+  c.dhz_c = -2.0 * p.mass * c.a2;
+  c.escape_radius = p.escape_radius;
+  c.capture_radius = p.capture_radius;
+  c.rtol = p.rtol;
+  c.atol = p.atol;
+  c.initial_step = p.initial_step;
+  c.max_step = p.max_step;
+  c.min_step = p.min_step;
+  c.max_steps = p.max_steps;
+  c.max_attempts = 4 * p.max_steps;
+  return c;
+}
+
+struct TraceIO {
+  const double *in;
+  int64_t ld_in;
+  double *out;
+  int64_t ld_out;
+  int32_t *status;
+  int64_t *steps;
+  int64_t *attempts;
+  unsigned long long *iterations;
+  int64_t n;
+};
+
+struct LaunchShape {
+  int block;
+  int rays_per_lane;
+  int refill_min;
+};
+
+// Host launchers implemented once per arithmetic build.
+#define BH_DECLARE_LAUNCHERS(NS)                                                   \
+  namespace NS {                                                                   \
+  hipError_t launch_trace_plain(const JobConst &c, const TraceIO &io,              \
+                                const LaunchShape &shape, hipStream_t s);          \
+  hipError_t launch_count_outside(const JobConst &c, const double *in, int64_t ld, \
+                                  int64_t n, int32_t *count, hipStream_t s);       \
+  hipError_t launch_geometry(const JobConst &c, const double *xyz, int64_t n,      \
+                             double *radius, double *h, double *l, double *gh,     \
+                             double *gl, hipStream_t s);                           \
+  hipError_t launch_rhs(const JobConst &c, const double *state, int64_t n,         \
+                        double *deriv, hipStream_t s);                             \
+  hipError_t launch_dp5_step(const JobConst &c, const double *state,               \
+                             const double *h, int64_t n, double *new_state,        \
+                             double *err, double *ratio, double *factor,           \
+                             hipStream_t s);                                       \
+  hipError_t launch_null_momentum(const JobConst &c, const double *pos,            \
+                                  const double *dir, int64_t n, int past,          \
+                                  int normalize, double *mom, hipStream_t s);      \
+  hipError_t launch_timelike_momentum(const JobConst &c, const double *pos,        \
+                                      const double *vel, int64_t n, double *mom,   \
+                                      int32_t *bad, hipStream_t s);                \
+  hipError_t launch_diagnostics(const JobConst &c, const double *state, int64_t n, \
+                                double *ham, double *energy, double *lz,           \
+                                double *vel, hipStream_t s);                       \
+  }
+
+BH_DECLARE_LAUNCHERS(precise)
+BH_DECLARE_LAUNCHERS(fast)
+
+hipError_t launch_aos_to_soa(const double *aos, double *soa, int64_t ld, int64_t n,
+                             hipStream_t s);
+hipError_t launch_soa_to_aos(const double *soa, int64_t ld, double *aos, int64_t n,
+                             hipStream_t s);
+hipError_t launch_zero_u64(unsigned long long *p, hipStream_t s);
+hipError_t launch_zero_i32(int32_t *p, hipStream_t s);
+
+}  // namespace bh

@@ -1,0 +1,160 @@
+/*
+ * bh_trace.h -- C ABI of the MI355X (gfx950) batched adaptive Kerr geodesic
+ * tracer.  All compute runs in hand-written HIP kernels, FP64 throughout.
+ *
+ * Conventions
+ *  - Every buffer is device memory owned by the caller (for example a
+ *    PyTorch-ROCm tensor passed by data_ptr()).  The library never allocates
+ *    caller-visible memory and never frees caller memory.
+ *  - Work is enqueued on the caller's HIP stream (`stream`, a hipStream_t;
+ *    NULL means the default stream).  No call synchronises the device.
+ *  - Return value: BH_OK (0) or a negative BH_E* code.  bh_last_error()
+ *    returns a thread-local message describing the last failure.
+ *  - "state" layout at the boundary: rows (t, x, y, z, p_t, p_x, p_y, p_z)
+ *    with covariant momenta (reference geodesics.py:3-8).  The tracer
+ *    kernels take structure-of-arrays planes: component c of ray i lives at
+ *    base[c * ld + i], ld >= n.  bh_aos_to_soa_f64/bh_soa_to_aos_f64 convert
+ *    from/to the (n, 8) row-major layout of the Python API.
+ *  - Ray status codes equal the reference's RayStatus (tracer.py:21-30).
+ *
+ * Citations are file:line in the reference repository (MegaKruk/BlackHorizon).
+ */
+#ifndef BH_TRACE_H
+#define BH_TRACE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BH_ABI_VERSION 1
+
+#define BH_OK 0
+#define BH_EINVAL -1   /* invalid argument (maps to ValueError) */
+#define BH_EHIP -2     /* HIP runtime / launch failure (maps to RuntimeError) */
+
+/* tracer.py:21-30 */
+#define BH_IN_FLIGHT 0
+#define BH_CAPTURED 1
+#define BH_ESCAPED 2
+#define BH_MAX_STEPS 3
+#define BH_FAILED 4
+#define BH_DISK 5
+#define BH_TERMINATED 6
+
+/* Arithmetic mode of a kernel.
+ * BH_MODE_FAST:    FMA contraction, shared reciprocals, Newton x^-0.2.
+ * BH_MODE_PRECISE: one correctly rounded IEEE operation per NumPy ufunc of
+ *                  the reference, in the reference's order (no contraction),
+ *                  libm-class pow; used to show rounding-level parity. */
+#define BH_MODE_FAST 0
+#define BH_MODE_PRECISE 1
+
+/* Controller settings of one trace job: the keyword arguments of
+ * trace_rays (tracer.py:50-61) plus the spacetime (kerr.py:61-73).
+ * capture_radius is r_+ (1 + capture_margin), computed by the caller exactly
+ * as tracer.py:84 does, so that the comparison at tracer.py:128-132 uses the
+ * same double. */
+typedef struct bh_trace_params {
+  double mass;
+  double spin;
+  double escape_radius;
+  double capture_radius;
+  double rtol;
+  double atol;
+  double initial_step;
+  double max_step;
+  double min_step;
+  int64_t max_steps;   /* accepted-step cap; attempts are capped at 4 * max_steps */
+  int32_t mode;        /* BH_MODE_FAST or BH_MODE_PRECISE */
+  int32_t reserved;
+} bh_trace_params;
+
+/* Launch shape of the plain tracer.  Zero fields pick defaults. */
+typedef struct bh_tune_params {
+  int32_t block_size;      /* threads per block, multiple of 64 (default 256) */
+  int32_t rays_per_lane;   /* rays each lane traces in turn (>= 1; default 1) */
+  int32_t refill_min;      /* idle lanes a wave collects before refilling (1..64) */
+  int32_t reserved;
+} bh_tune_params;
+
+int32_t bh_abi_version(void);
+const char *bh_last_error(void);
+
+/* Number of rays with r0 >= escape_radius, written to *count_out (device
+ * int32).  Replaces the initial-radius check of tracer.py:94-99; the host
+ * raises ValueError when the count is nonzero. */
+int32_t bh_count_outside_f64(const bh_trace_params *p, const double *state_soa,
+                             int64_t ld, int64_t n, int32_t *count_out,
+                             void *stream);
+
+/* Trace n rays to termination: one DP5(4) attempt per loop pass per ray,
+ * exactly the controller of tracer.py:101-154 applied per ray.  Outputs:
+ * final states (SoA), status (int32), accepted steps (int64), attempts
+ * (int64, accepted + rejected), and *iterations (device uint64) = max
+ * attempts over the rays (tracer.py:155-157 `iterations`).
+ * Replaces trace_rays (tracer.py:50-157). */
+int32_t bh_trace_plain_f64(const bh_trace_params *p, const bh_tune_params *tune,
+                           const double *state_in_soa, int64_t ld_in,
+                           double *state_out_soa, int64_t ld_out,
+                           int32_t *status, int64_t *steps, int64_t *attempts,
+                           uint64_t *iterations, int64_t n, void *stream);
+
+/* (n, 8) row-major <-> 8 planes of ld.  Replaces nothing in the reference
+ * (its arrays stay (n, 8)); the boundary transpose of the SoA design. */
+int32_t bh_aos_to_soa_f64(const double *aos, double *soa, int64_t ld, int64_t n,
+                          void *stream);
+int32_t bh_soa_to_aos_f64(const double *soa, int64_t ld, double *aos, int64_t n,
+                          void *stream);
+
+/* KerrSpacetime.geometry(x, y, z, gradients) (kerr.py:118-193) at n points
+ * given as (n, 3) rows.  Outputs (any may be NULL when not wanted):
+ * radius (n), h (n), l (n, 3), grad_h (n, 3), grad_l (n, 3, 3) with
+ * grad_l[i][j] = d l_j / d x_i. */
+int32_t bh_geometry_f64(double mass, double spin, int32_t mode,
+                        const double *xyz, int64_t n, double *radius, double *h,
+                        double *l, double *grad_h, double *grad_l, void *stream);
+
+/* geodesic_rhs (geodesics.py:42-74): (n, 8) rows -> (n, 8) rows. */
+int32_t bh_geodesic_rhs_f64(double mass, double spin, int32_t mode,
+                            const double *state, int64_t n, double *deriv,
+                            void *stream);
+
+/* dormand_prince_step (integrators.py:76-107) with the geodesic RHS, then
+ * error_ratio (110-117) and step_factor (120-124): (n, 8) rows and h (n) ->
+ * new_state (n, 8), error (n, 8), ratio (n), factor (n).  ratio/factor may
+ * be NULL. */
+int32_t bh_dp5_step_f64(double mass, double spin, int32_t mode, double rtol,
+                        double atol, const double *state, const double *h,
+                        int64_t n, double *new_state, double *error,
+                        double *ratio, double *factor, void *stream);
+
+/* null_momentum_from_velocity (geodesics.py:109-170): positions (n, 3),
+ * directions (n, 3) -> covariant momenta (n, 4).  past != 0 picks the
+ * past-directed root; normalize != 0 rescales to |p_t| = 1. */
+int32_t bh_null_momentum_f64(double mass, double spin, const double *positions,
+                             const double *directions, int64_t n, int32_t past,
+                             int32_t normalize, double *momenta, void *stream);
+
+/* timelike_momentum (geodesics.py:173-209): positions (n, 3), dx^i/dt (n, 3)
+ * -> momenta (n, 4); *bad_count (device int32) receives the number of
+ * non-timelike inputs (the host raises ValueError when nonzero). */
+int32_t bh_timelike_momentum_f64(double mass, double spin,
+                                 const double *positions,
+                                 const double *velocities, int64_t n,
+                                 double *momenta, int32_t *bad_count,
+                                 void *stream);
+
+/* hamiltonian (geodesics.py:77-85), conserved_quantities (88-92) and
+ * coordinate_velocity (95-106) of (n, 8) rows.  Any output may be NULL:
+ * ham (n), energy (n), l_z (n), velocity (n, 4). */
+int32_t bh_diagnostics_f64(double mass, double spin, const double *state,
+                           int64_t n, double *ham, double *energy, double *l_z,
+                           double *velocity, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BH_TRACE_H */

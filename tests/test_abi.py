@@ -1,0 +1,76 @@
+"""The C-ABI library loads, exports every symbol include/bh_trace.h
+declares, and rejects invalid arguments before touching a device."""
+
+import ctypes
+
+import pytest
+
+from blackhorizon_amd import _lib
+
+
+def test_library_exports_header_symbols():
+    lib = _lib.load()
+    declared = _lib.header_symbols()
+    assert len(declared) >= 12
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert set(declared) == set(_lib.SIGNATURES)
+    assert lib.bh_abi_version() == _lib.ABI_VERSION
+
+
+def _params(**over):
+    p = dict(mass=1.0, spin=0.9, escape_radius=45.0, capture_radius=1.5, rtol=1e-8, atol=1e-10,
+             initial_step=0.1, max_step=2.0, min_step=1e-10, max_steps=20000, mode=0, reserved=0)
+    p.update(over)
+    return _lib.TraceParams(**p)
+
+
+@pytest.mark.parametrize("over,msg", [
+    (dict(spin=1.01), "Kerr bound"),
+    (dict(mass=-1.0), "mass must be positive"),
+    (dict(mode=7), "mode"),
+])
+def test_invalid_params_rejected(over, msg):
+    lib = _lib.load()
+    rc = lib.bh_trace_plain_f64(_params(**over), None, None, 0, None, 0, None, None, None, None,
+                                0, None)
+    assert rc == _lib.BH_EINVAL
+    assert msg in lib.bh_last_error().decode()
+    with pytest.raises(ValueError):
+        _lib.check(rc, "trace")
+
+
+def test_invalid_tune_rejected():
+    lib = _lib.load()
+    for tune in (_lib.TuneParams(96, 1, 1, 0), _lib.TuneParams(512, 1, 1, 0),
+                 _lib.TuneParams(256, -1, 1, 0), _lib.TuneParams(256, 1, 65, 0)):
+        rc = lib.bh_trace_plain_f64(_params(), tune, None, 0, None, 0, None, None, None, None, 0,
+                                    None)
+        assert rc == _lib.BH_EINVAL
+
+
+def test_bad_sizes_rejected():
+    lib = _lib.load()
+    assert lib.bh_trace_plain_f64(_params(), None, None, 5, None, 5, None, None, None, None, 10,
+                                  None) == _lib.BH_EINVAL
+    assert lib.bh_aos_to_soa_f64(None, None, 3, 4, None) == _lib.BH_EINVAL
+    assert lib.bh_geometry_f64(1.0, 2.0, 0, None, 0, None, None, None, None, None, None) \
+        == _lib.BH_EINVAL
+
+
+def test_python_api_errors_without_device(monkeypatch):
+    import torch
+
+    import blackhorizon_amd as bh
+
+    if torch.cuda.is_available():
+        pytest.skip("device present")
+    with pytest.raises(RuntimeError):
+        bh.trace_rays(bh.KerrSpacetime(spin=0.5), [[0.0, 10.0, 0, 0, 1.0, -1.0, 0, 0]], 20.0)
+    with pytest.raises(ValueError):
+        bh.KerrSpacetime(spin=1.5)
+    assert not bh.gpu_available()
+    with pytest.raises(RuntimeError):
+        bh.get_xp("hip")
+    with pytest.raises(ValueError):
+        bh.get_xp("cuda")
