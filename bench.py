@@ -160,11 +160,11 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="hd", choices=["hd", "hd09", "small"])
-    ap.add_argument("--mode", default="fast", choices=["fast", "precise"])
+    ap.add_argument("--mode", default="auto", choices=["auto", "fast", "precise"])
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--rays-per-lane", type=int, default=1)
     ap.add_argument("--refill-min", type=int, default=1)
-    ap.add_argument("--cpu-rays-per-frame", type=int, default=2048)
+    ap.add_argument("--cpu-rays-per-frame", type=int, default=6144)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = ap.parse_args(argv)
 
@@ -275,8 +275,8 @@ def main(argv=None) -> int:
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP64_VECTOR_PEAK_TFLOPS,
             "traffic": None,
-            "kernel": "trace_plain_kernel (fast)" if args.mode == "fast"
-                      else "trace_plain_kernel (precise)",
+            "kernel": "trace_plain_kernel ("
+                      + tracer.resolve_mode(args.mode, jobs[0].rtol) + " build)",
             "flop_per_launch": flop / len(events),
             "mean_launch_ms": float(np.mean(kern_ms)),
         },

@@ -54,14 +54,29 @@ class Tune:
         return _lib.TuneParams(self.block_size, self.rays_per_lane, self.refill_min, 0)
 
 
+# Below this rtol the fast build's last-ulp RHS rounding, amplified by the
+# embedded error estimate (~1e-16 / rtol), moves final states by more than
+# 1e-7 relative to the reference; "auto" then uses the precise build.
+FAST_MIN_RTOL = 1e-9
+
+
+def resolve_mode(mode: str, rtol: float) -> str:
+    if mode == "auto":
+        return "fast" if rtol >= FAST_MIN_RTOL else "precise"
+    if mode not in ("fast", "precise"):
+        raise ValueError("mode must be 'auto', 'fast' or 'precise'")
+    return mode
+
+
 def make_params(spacetime: KerrSpacetime, escape_radius: float, max_steps: int, rtol: float,
                 atol: float, initial_step: float, max_step: float, min_step: float,
-                capture_margin: float, mode: str = "fast") -> _lib.TraceParams:
+                capture_margin: float, mode: str = "auto") -> _lib.TraceParams:
     # tracer.py:84 computes the capture radius in host double precision
     capture_radius = spacetime.outer_horizon_radius * (1.0 + capture_margin)
     return _lib.TraceParams(spacetime.mass, spacetime.spin, float(escape_radius),
                             capture_radius, float(rtol), float(atol), float(initial_step),
-                            float(max_step), float(min_step), int(max_steps), _mode(mode), 0)
+                            float(max_step), float(min_step), int(max_steps),
+                            _mode(resolve_mode(mode, rtol)), 0)
 
 
 def trace_rays(
@@ -76,7 +91,7 @@ def trace_rays(
     min_step: float = 1e-10,
     capture_margin: float = 1e-3,
     *,
-    mode: str = "fast",
+    mode: str = "auto",
     tune: Tune | None = None,
 ) -> TraceResult:
     """Trace a batch of geodesics to termination (tracer.py:50-157).
@@ -84,8 +99,11 @@ def trace_rays(
     ``state0`` is an (n, 8) NumPy array (copied to the device; results come
     back as NumPy) or a float64 device tensor (results stay on the device).
     It is never modified.  Raises ValueError if any initial Kerr-Schild
-    radius is >= ``escape_radius`` (tracer.py:94-99).  ``mode="precise"``
-    selects the build that rounds like the reference's NumPy expressions.
+    radius is >= ``escape_radius`` (tracer.py:94-99).  ``mode``: "precise"
+    rounds every operation as the reference's NumPy expressions do; "fast"
+    contracts to FMAs and shares reciprocals; "auto" (default) takes fast
+    for rtol >= 1e-9 and precise below, where the fast build's rounding
+    would move final states by more than 1e-7.
     """
     params = make_params(spacetime, escape_radius, max_steps, rtol, atol, initial_step,
                          max_step, min_step, capture_margin, mode)
