@@ -112,36 +112,74 @@ def cpu_baseline(jobs, per_frame: int, seed: int = 7, processes: int = 1):
 # GPU
 # --------------------------------------------------------------------------
 
-def prepare_gpu(jobs, mode):
+def camera_order(job, order: str):
+    if order == "row":
+        return None
+    from blackhorizon_amd.camera import PinholeCamera
+
+    cam = PinholeCamera.from_orbit(job.distance, job.inclination_degrees,
+                                   azimuth_degrees=job.azimuth_degrees,
+                                   fov_degrees=job.fov_degrees, width=job.width,
+                                   height=job.height)
+    return cam.tile_order(int(order[4:]) if order.startswith("tile") and order[4:] else 8)
+
+
+def device_state(job):
+    """(n, 8) camera rays of a job built on the device."""
     import torch
 
-    from blackhorizon_amd import _lib, geodesics, kerr, tracer
+    from blackhorizon_amd import geodesics, kerr
+
+    dev = torch.device("cuda", 0)
+    st = kerr.KerrSpacetime(job.mass, job.spin)
+    pos, dirs = camera_arrays(job)
+    pos_t = torch.from_numpy(pos).to(dev)
+    mom = geodesics.null_momentum_from_velocity(st, pos_t, torch.from_numpy(dirs).to(dev), "past")
+    return st, geodesics.build_state(pos_t, mom)
+
+
+def prepare_gpu(jobs, mode, order="row"):
+    import torch
+
+    from blackhorizon_amd import _lib, tracer
 
     lib = _lib.require_device()
     dev = torch.device("cuda", 0)
     stream = _lib.stream_handle()
     prepared = []
     for job in jobs:
-        st = kerr.KerrSpacetime(job.mass, job.spin)
-        pos, dirs = camera_arrays(job)
-        pos_t = torch.from_numpy(pos).to(dev)
-        mom = geodesics.null_momentum_from_velocity(st, pos_t, torch.from_numpy(dirs).to(dev),
-                                                    "past")
-        state = geodesics.build_state(pos_t, mom)
+        st, state = device_state(job)
         n = job.n_rays
         soa_in = torch.empty((8, n), dtype=torch.float64, device=dev)
         _lib.check(lib.bh_aos_to_soa_f64(_lib.ptr(state), _lib.ptr(soa_in), n, n, stream), "soa")
+        del state
         params = tracer.make_params(st, job.escape_radius, job.max_steps, job.rtol, job.atol,
                                     job.initial_step, job.max_step, job.min_step,
                                     job.capture_margin, mode)
+        o = camera_order(job, order)
         bufs = dict(soa_in=soa_in, soa_out=torch.empty_like(soa_in),
                     status=torch.empty(n, dtype=torch.int32, device=dev),
                     steps=torch.empty(n, dtype=torch.int64, device=dev),
                     attempts=torch.empty(n, dtype=torch.int64, device=dev),
-                    iters=torch.zeros(1, dtype=torch.int64, device=dev))
+                    iters=torch.zeros(1, dtype=torch.int64, device=dev),
+                    order=None if o is None else torch.from_numpy(o).to(dev))
         prepared.append((job, params, bufs))
     torch.cuda.synchronize()
     return prepared
+
+
+def lane_efficiency(b, rays_per_lane):
+    """Fraction of lane-attempts doing useful work when every 64 consecutive
+    rays of the hand-out order share a wavefront (exact for rays_per_lane 1)."""
+    import torch
+
+    a = b["attempts"] if b.get("order") is None else b["attempts"][b["order"]]
+    n = a.shape[0]
+    pad = (-n) % 64
+    if pad:
+        a = torch.cat([a, torch.zeros(pad, dtype=a.dtype, device=a.device)])
+    w = a.view(-1, 64)
+    return float(a.sum().item()) / float((64 * w.max(dim=1).values).sum().item())
 
 
 def launch(lib, params, tune, b, stream):
@@ -151,7 +189,74 @@ def launch(lib, params, tune, b, stream):
     _lib.check(lib.bh_trace_plain_f64(params, tune, _lib.ptr(b["soa_in"]), n,
                                       _lib.ptr(b["soa_out"]), n, _lib.ptr(b["status"]),
                                       _lib.ptr(b["steps"]), _lib.ptr(b["attempts"]),
-                                      _lib.ptr(b["iters"]), n, stream), "trace")
+                                      _lib.ptr(b["iters"]), _lib.ptr(b.get("order")), n, stream),
+               "trace")
+
+
+def run_uneven(args) -> dict:
+    """The uneven 64-job mix: one persistent batched launch per arithmetic
+    mode (fast jobs and precise jobs on two streams) against one plain
+    launch per job on one stream.  Same rays, same per-ray results."""
+    import torch
+
+    from blackhorizon_amd import _lib, batch, tracer, workloads
+
+    lib = _lib.require_device()
+    jobs = workloads.uneven_mix()
+    tjobs = []
+    for job in jobs:
+        st, state = device_state(job)
+        tjobs.append(batch.TraceJob(st, state, job.escape_radius, job.max_steps, job.rtol,
+                                    job.atol, job.initial_step, job.max_step, job.min_step,
+                                    job.capture_margin))
+    tune = tracer.Tune(args.block, 1, args.refill_min)
+    plan = batch.BatchPlan(tjobs, mode=args.mode, tune=tune, concurrent=not args.serial_groups)
+    prepared = prepare_gpu(jobs, args.mode, "row")
+    for t in tjobs:
+        t.state0 = None
+    torch.cuda.synchronize()
+    cur = torch.cuda.current_stream()
+    stream = _lib.stream_handle()
+    tc = tune.as_c()
+
+    def timed(fn, k):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(cur)
+        for _ in range(k):
+            fn()
+        e1.record(cur)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / k
+
+    def per_job():
+        for job, params, b in prepared:
+            launch(lib, params, tc, b, stream)
+
+    for _ in range(args.warmup):
+        plan.launch()
+        per_job()
+    torch.cuda.synchronize()
+    ms_batched = timed(plan.launch, args.steps)
+    print("[bench] batched done", file=sys.stderr, flush=True)
+    ms_per_job = timed(per_job, args.steps)
+    print("[bench] per-job done", file=sys.stderr, flush=True)
+    steps_b = sum(int(g.steps.sum().item()) for g in plan.groups)
+    att_b = sum(int(g.attempts.sum().item()) for g in plan.groups)
+    steps_p = sum(int(b["steps"].sum().item()) for _, _, b in prepared)
+    same = all(torch.equal(g.steps, torch.cat([prepared[i][2]["steps"] for i in g.idx]))
+               for g in plan.groups)
+    modes = {g.mode: len(g.idx) for g in plan.groups}
+    return dict(
+        batched_ray_steps_per_s=steps_b / (ms_batched * 1e-3),
+        per_job_ray_steps_per_s=steps_p / (ms_per_job * 1e-3),
+        speedup=ms_per_job / ms_batched,
+        ms_batched=ms_batched, ms_per_job=ms_per_job,
+        ray_steps=steps_b, attempts=att_b, rays=plan.total_rays,
+        pct_fp64_vector_peak_batched=100.0 * att_b * FLOP_PER_ATTEMPT / (ms_batched * 1e-3)
+        / (FP64_VECTOR_PEAK_TFLOPS * 1e12),
+        jobs_per_mode=modes, per_ray_steps_identical=bool(same and steps_b == steps_p))
 
 
 def main(argv=None) -> int:
@@ -159,11 +264,15 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="hd", choices=["hd", "hd09", "small"])
+    ap.add_argument("--workload", default="hd", choices=["hd", "hd09", "small", "uneven"])
+    ap.add_argument("--serial-groups", action="store_true",
+                    help="uneven: run the fast and precise batches one after the other")
     ap.add_argument("--mode", default="auto", choices=["auto", "fast", "precise"])
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--rays-per-lane", type=int, default=1)
     ap.add_argument("--refill-min", type=int, default=1)
+    ap.add_argument("--order", default="row",
+                    help="ray hand-out order: row (pixel order) or tile<k> (k x k pixel tiles)")
     ap.add_argument("--cpu-rays-per-frame", type=int, default=6144)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = ap.parse_args(argv)
@@ -174,6 +283,12 @@ def main(argv=None) -> int:
                                    "exchange, but only the single-GPU path is implemented and "
                                    "measured; run with --gpus 1"}))
         return 2
+
+    if args.workload == "uneven":
+        res = run_uneven(args)
+        print(json.dumps({"metric": METRIC, "workload": "uneven 64-job mix", "mode": args.mode,
+                          **res}), flush=True)
+        return 0
 
     jobs = frames(args.workload)
     cpu = cpu_par = None
@@ -190,7 +305,7 @@ def main(argv=None) -> int:
     from blackhorizon_amd import _lib, tracer
 
     lib = _lib.require_device()
-    prepared = prepare_gpu(jobs, args.mode)
+    prepared = prepare_gpu(jobs, args.mode, args.order)
     tune = tracer.Tune(args.block, args.rays_per_lane, args.refill_min).as_c()
     stream = _lib.stream_handle()
     cur = torch.cuda.current_stream()
@@ -224,6 +339,7 @@ def main(argv=None) -> int:
         st = torch.bincount(b["status"].long(), minlength=7).tolist()
         ms = [kern_ms[k * len(prepared) + i] for k in range(args.steps)]
         per_frame.append(dict(spin=job.spin, inclination=job.inclination_degrees,
+                              lane_efficiency=round(lane_efficiency(b, args.rays_per_lane), 4),
                               rays=job.n_rays, ray_steps=s, attempts=a,
                               iterations=int(b["iters"].item()),
                               kernel_ms=round(float(np.mean(ms)), 3),
@@ -263,6 +379,7 @@ def main(argv=None) -> int:
             "mode": args.mode,
             "tune": {"block": args.block, "rays_per_lane": args.rays_per_lane,
                      "refill_min": args.refill_min},
+            "order": args.order,
         },
         "ns_per_ray_step": wall / (steps_sweep * args.steps) * 1e9,
         "attempts_per_s": attempts_sweep * args.steps / wall,

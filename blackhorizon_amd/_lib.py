@@ -67,7 +67,9 @@ SIGNATURES = {
     "bh_last_error": [],
     "bh_count_outside_f64": [ctypes.POINTER(TraceParams), _P, _I64, _I64, _P, _P],
     "bh_trace_plain_f64": [ctypes.POINTER(TraceParams), ctypes.POINTER(TuneParams), _P, _I64,
-                           _P, _I64, _P, _P, _P, _P, _I64, _P],
+                           _P, _I64, _P, _P, _P, _P, _P, _I64, _P],
+    "bh_trace_batched_f64": [_P, _P, _I32, _I32, ctypes.POINTER(TuneParams), _P, _I64, _P, _I64,
+                             _P, _P, _P, _P, _P, _P, _I64, _P],
     "bh_aos_to_soa_f64": [_P, _P, _I64, _I64, _P],
     "bh_soa_to_aos_f64": [_P, _I64, _P, _I64, _P],
     "bh_geometry_f64": [_D, _D, _I32, _P, _I64, _P, _P, _P, _P, _P, _P],

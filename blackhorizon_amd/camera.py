@@ -80,3 +80,20 @@ class PinholeCamera:
         """Camera position per pixel, (height * width, 3) (camera.py:115-120)."""
         o = numpy.asarray(self.position, dtype=numpy.float64)
         return numpy.broadcast_to(o, (self.height * self.width, 3)).copy()
+
+    def tile_order(self, tile: int = 8) -> numpy.ndarray:
+        """Permutation of ray indices that lists the pixels tile by tile
+        (tile x tile blocks, row-major inside and across tiles; partial
+        tiles at the right and bottom edges included).  Handing rays to
+        64-lane wavefronts in this order groups neighbouring pixels, whose
+        trace lengths are similar, into the same wavefront."""
+        if tile < 1:
+            raise ValueError("tile must be >= 1")
+        rows = numpy.arange(self.height)
+        cols = numpy.arange(self.width)
+        r, c = numpy.meshgrid(rows, cols, indexing="ij")
+        tiles_per_row = (self.width + tile - 1) // tile
+        key = ((r // tile) * tiles_per_row + (c // tile)) * (tile * tile) \
+            + (r % tile) * tile + (c % tile)
+        return numpy.argsort(key.ravel(), kind="stable").astype(numpy.int64)
+

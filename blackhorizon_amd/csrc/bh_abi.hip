@@ -117,7 +117,8 @@ int32_t bh_count_outside_f64(const bh_trace_params *p, const double *state_soa, 
 int32_t bh_trace_plain_f64(const bh_trace_params *p, const bh_tune_params *tune,
                            const double *state_in_soa, int64_t ld_in, double *state_out_soa,
                            int64_t ld_out, int32_t *status, int64_t *steps, int64_t *attempts,
-                           uint64_t *iterations, int64_t n, void *stream) {
+                           uint64_t *iterations, const int64_t *order, int64_t n,
+                           void *stream) {
   int32_t rc = check_params(p);
   if (rc) return rc;
   if (n < 0 || ld_in < n || ld_out < n) return fail(BH_EINVAL, "bad n / ld");
@@ -137,13 +138,47 @@ int32_t bh_trace_plain_f64(const bh_trace_params *p, const bh_tune_params *tune,
   hipStream_t s = (hipStream_t)stream;
   JobConst c = make_job_const(*p);
   TraceIO io{state_in_soa, ld_in, state_out_soa, ld_out, status, steps, attempts,
-             (unsigned long long *)iterations, n};
+             (unsigned long long *)iterations, order, n};
   hipError_t e = hipSuccess;
   if (iterations) e = launch_zero_u64((unsigned long long *)iterations, s);
   if (e == hipSuccess)
     e = p->mode == BH_MODE_PRECISE ? precise::launch_trace_plain(c, io, shape, s)
                                    : fast::launch_trace_plain(c, io, shape, s);
   return hip_status(e, "bh_trace_plain_f64");
+}
+
+int32_t bh_trace_batched_f64(const bh_trace_params *jobs, const int64_t *ray_offsets,
+                             int32_t n_jobs, int32_t mode, const bh_tune_params *tune,
+                             const double *state_in_soa, int64_t ld_in, double *state_out_soa,
+                             int64_t ld_out, int32_t *status, int64_t *steps, int64_t *attempts,
+                             uint64_t *iterations, const int64_t *order, uint64_t *workspace,
+                             int64_t n_total, void *stream) {
+  if (n_jobs < 1 || n_jobs > MAX_BATCH_JOBS)
+    return fail(BH_EINVAL, "n_jobs must be in [1, 256]");
+  if (mode != BH_MODE_FAST && mode != BH_MODE_PRECISE)
+    return fail(BH_EINVAL, "mode must be BH_MODE_FAST or BH_MODE_PRECISE");
+  if (n_total < 0 || ld_in < n_total || ld_out < n_total) return fail(BH_EINVAL, "bad n / ld");
+  if (!jobs || !ray_offsets || !iterations || !workspace) return fail(BH_EINVAL, "NULL buffer");
+  if (n_total > 0 && (!state_in_soa || !state_out_soa || !status || !steps || !attempts))
+    return fail(BH_EINVAL, "NULL buffer");
+  LaunchShape shape{256, 1, 1};
+  if (tune) {
+    if (tune->block_size) shape.block = tune->block_size;
+    if (tune->refill_min) shape.refill_min = tune->refill_min;
+  }
+  if (shape.block <= 0 || shape.block > 256 || shape.block % 64)
+    return fail(BH_EINVAL, "block_size must be a multiple of 64 in [64, 256]");
+  if (shape.refill_min < 1 || shape.refill_min > 64)
+    return fail(BH_EINVAL, "refill_min must be in [1, 64]");
+  TraceIO io{state_in_soa, ld_in, state_out_soa, ld_out, status, steps, attempts,
+             (unsigned long long *)iterations, order, n_total};
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = mode == BH_MODE_PRECISE
+                     ? precise::launch_trace_batched(jobs, ray_offsets, n_jobs, io,
+                                                     (unsigned long long *)workspace, shape, s)
+                     : fast::launch_trace_batched(jobs, ray_offsets, n_jobs, io,
+                                                  (unsigned long long *)workspace, shape, s);
+  return hip_status(e, "bh_trace_batched_f64");
 }
 
 int32_t bh_aos_to_soa_f64(const double *aos, double *soa, int64_t ld, int64_t n, void *stream) {

@@ -25,7 +25,7 @@ struct JobConst {
   int64_t max_attempts;  // 4 * max_steps  (tracer.py:101)
 };
 
-inline JobConst make_job_const(const bh_trace_params &p) {
+__host__ __device__ inline JobConst make_job_const(const bh_trace_params &p) {
   JobConst c;
   c.m = p.mass;
   c.a = p.spin;
@@ -54,8 +54,12 @@ struct TraceIO {
   int64_t *steps;
   int64_t *attempts;
   unsigned long long *iterations;
+  const int64_t *order;  // optional: k-th ray handed out is order[k]
   int64_t n;
 };
+
+// Jobs one batched launch can hold (job table and ray offsets live in LDS).
+constexpr int MAX_BATCH_JOBS = 256;
 
 struct LaunchShape {
   int block;
@@ -68,6 +72,10 @@ struct LaunchShape {
   namespace NS {                                                                   \
   hipError_t launch_trace_plain(const JobConst &c, const TraceIO &io,              \
                                 const LaunchShape &shape, hipStream_t s);          \
+  hipError_t launch_trace_batched(const bh_trace_params *jobs, const int64_t *offsets,   \
+                                  int n_jobs, const TraceIO &io,                   \
+                                  unsigned long long *queue, const LaunchShape &shape, \
+                                  hipStream_t s);                                  \
   hipError_t launch_count_outside(const JobConst &c, const double *in, int64_t ld, \
                                   int64_t n, int32_t *count, hipStream_t s);       \
   hipError_t launch_geometry(const JobConst &c, const double *xyz, int64_t n,      \

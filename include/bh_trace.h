@@ -94,12 +94,40 @@ int32_t bh_count_outside_f64(const bh_trace_params *p, const double *state_soa,
  * final states (SoA), status (int32), accepted steps (int64), attempts
  * (int64, accepted + rejected), and *iterations (device uint64) = max
  * attempts over the rays (tracer.py:155-157 `iterations`).
+ * order: optional device int64[n] permutation of [0, n): rays are handed to
+ * wavefront lanes in this order (results still land at each ray's own
+ * index); NULL = identity.  Results do not depend on order or tune.
  * Replaces trace_rays (tracer.py:50-157). */
 int32_t bh_trace_plain_f64(const bh_trace_params *p, const bh_tune_params *tune,
                            const double *state_in_soa, int64_t ld_in,
                            double *state_out_soa, int64_t ld_out,
                            int32_t *status, int64_t *steps, int64_t *attempts,
-                           uint64_t *iterations, int64_t n, void *stream);
+                           uint64_t *iterations, const int64_t *order, int64_t n,
+                           void *stream);
+
+/* Persistent batched trace of n_jobs independent jobs in one launch.
+ * jobs: device array of n_jobs bh_trace_params (each job's own spacetime,
+ *   camera-independent controller settings; the `mode` field is ignored --
+ *   the launch's `mode` selects the arithmetic build for all jobs).  Jobs must
+ *   satisfy the checks bh_trace_plain_f64 applies to its params.
+ * ray_offsets: device int64[n_jobs + 1], nondecreasing, ray_offsets[0] = 0,
+ *   ray_offsets[n_jobs] = n_total: job j owns rays [off[j], off[j + 1]) of
+ *   the concatenated SoA planes.
+ * iterations: device uint64[n_jobs], per-job max attempts (tracer.py:155-157).
+ * order: optional device int64[n_total] permutation: the queue hands out
+ *   ray order[k] k-th (NULL = job order).
+ * workspace: device uint64[1] scratch (the ray-queue counter).
+ * Outputs per ray exactly as bh_trace_plain_f64 would give for each job
+ * alone; only the schedule differs (lanes refill from a global ray queue as
+ * rays retire).  n_jobs <= 256.  Replaces a loop of trace_rays calls
+ * (tracer.py:50-157), one per job. */
+int32_t bh_trace_batched_f64(const bh_trace_params *jobs, const int64_t *ray_offsets,
+                             int32_t n_jobs, int32_t mode, const bh_tune_params *tune,
+                             const double *state_in_soa, int64_t ld_in,
+                             double *state_out_soa, int64_t ld_out, int32_t *status,
+                             int64_t *steps, int64_t *attempts, uint64_t *iterations,
+                             const int64_t *order, uint64_t *workspace, int64_t n_total,
+                             void *stream);
 
 /* (n, 8) row-major <-> 8 planes of ld.  Replaces nothing in the reference
  * (its arrays stay (n, 8)); the boundary transpose of the SoA design. */

@@ -347,6 +347,22 @@ def step_factor(ratio):
     return np.clip(0.9 * safe ** (-0.2), 0.2, 5.0)
 
 
+def step_factor_libm(ratio):
+    """step_factor with the C library's pow instead of NumPy's power ufunc.
+
+    NumPy's float64 power dispatches to a vendor SIMD pow on AVX-512 hosts
+    and to libm elsewhere; the two differ by an ulp in a few percent of
+    calls.  Tracing with this variant measures how far the reference's own
+    results move between such hosts (its conditioning), which is the scale
+    any other correct implementation can be held to on ill-conditioned rays.
+    """
+    import math
+
+    safe = np.maximum(ratio, 1e-16)
+    p = np.array([math.pow(float(v), -0.2) for v in np.ravel(safe)]).reshape(np.shape(safe))
+    return np.clip(0.9 * p, 0.2, 5.0)
+
+
 def integrate_dense(rhs, state0, lambda_total, rtol=1e-10, atol=1e-12,
                     initial_step=0.05, max_step=2.0):
     """Batch-shared adaptive DP5 over an affine interval; integrators.py:127-174."""
@@ -400,13 +416,15 @@ class TraceResult:
 
 def trace_rays(st: Kerr, state0, escape_radius, max_steps=20000, rtol=1e-9,
                atol=1e-12, initial_step=0.1, max_step=2.0, min_step=1e-10,
-               capture_margin=1e-3, return_attempts: bool = False):
+               capture_margin=1e-3, return_attempts: bool = False, factor_fn=None):
     """Active-set DP5 controller with termination; tracer.py:50-157.
 
     ``return_attempts`` additionally returns the per-ray attempt counts
     (accepted + rejected), which the reference does not expose but which
     equal the number of loop iterations each ray was active for.
+    ``factor_fn`` replaces step_factor (see step_factor_libm).
     """
+    factor_fn = step_factor if factor_fn is None else factor_fn
     n = state0.shape[0]
     capture_radius = st.outer_horizon_radius * (1.0 + capture_margin)
     states = state0.copy()
@@ -439,7 +457,7 @@ def trace_rays(st: Kerr, state0, escape_radius, max_steps=20000, rtol=1e-9,
         states[active] = y
         steps_a = steps[active] + accept.astype(steps.dtype)
         steps[active] = steps_a
-        h_a = np.clip(h_a * step_factor(ratio), 0.0, max_step)
+        h_a = np.clip(h_a * factor_fn(ratio), 0.0, max_step)
         h[active] = h_a
         radius = st.kerr_schild_radius(y[:, 1], y[:, 2], y[:, 3])
         finite = np.all(np.isfinite(y), axis=-1)

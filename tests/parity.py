@@ -46,14 +46,48 @@ def max_rel_diff(got, ref):
     return worst
 
 
+def block_diff(got, ref):
+    """Per-ray largest block-norm relative difference (see state_close)."""
+    out = np.zeros(np.asarray(ref).shape[0])
+    for g, r in zip(_blocks(got), _blocks(ref)):
+        with np.errstate(invalid="ignore"):
+            d = np.linalg.norm(g - r, axis=-1) / np.maximum(np.linalg.norm(r, axis=-1), 1.0)
+        same = np.all((np.isnan(g) & np.isnan(r)) | (g == r), axis=-1)
+        out = np.maximum(out, np.where(same, 0.0, np.nan_to_num(d, nan=np.inf)))
+    return out
+
+
+def reference_spread(spin, state0, ref_states, **trace_kw):
+    """How far the reference's own final states move when NumPy's power ufunc
+    is replaced by libm pow (a different but equally valid host): traced by
+    the oracle, which reproduces the reference bit for bit on this host."""
+    from oracle import bh_oracle as orc
+    import warnings
+
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)
+        alt = orc.trace_rays(orc.Kerr(spin=spin), state0, factor_fn=orc.step_factor_libm,
+                             **trace_kw)
+    return block_diff(alt.states, ref_states)
+
+
+SPREAD_FACTOR = 10.0
+
+
 def assert_trace_parity(got_states, got_status, got_steps, ref_states, ref_status, ref_steps,
-                        allow_mismatch=0, label=""):
-    """Status and accepted-step counts identical, final states within
-    STATE_RTOL; up to ``allow_mismatch`` rays may differ (rays on the
-    critical curve, where round-off decides the outcome)."""
+                        allow_mismatch=0, label="", spread=None):
+    """Status and accepted-step counts identical; final states within
+    STATE_RTOL, or within SPREAD_FACTOR x the reference's own host-to-host
+    spread on rays where that spread exceeds STATE_RTOL (captured rays that
+    end inside the horizon with momenta of order 1e3 and shrinking steps are
+    that ill-conditioned); up to ``allow_mismatch`` rays may differ."""
     st_ok = np.asarray(got_status) == np.asarray(ref_status)
     steps_ok = np.asarray(got_steps) == np.asarray(ref_steps)
-    s_ok = state_close(got_states, ref_states)
+    tol = np.full(np.asarray(ref_status).shape[0], STATE_RTOL)
+    if spread is not None:
+        tol = np.maximum(tol, SPREAD_FACTOR * np.asarray(spread))
+    d = block_diff(got_states, ref_states)
+    s_ok = d <= tol
     bad = ~(st_ok & steps_ok & s_ok)
     nbad = int(bad.sum())
     if nbad > allow_mismatch:
@@ -62,6 +96,6 @@ def assert_trace_parity(got_states, got_status, got_steps, ref_states, ref_statu
             f"{label}: {nbad} rays differ (allowed {allow_mismatch}); first {idx.tolist()}; "
             f"status {np.asarray(got_status)[idx].tolist()} vs {np.asarray(ref_status)[idx].tolist()}; "
             f"steps {np.asarray(got_steps)[idx].tolist()} vs {np.asarray(ref_steps)[idx].tolist()}; "
-            f"max rel state diff over agreeing rays "
-            f"{max_rel_diff(np.asarray(got_states)[~bad], np.asarray(ref_states)[~bad]):.3e}")
+            f"state diff {d[idx].tolist()} tol {tol[idx].tolist()}; "
+            f"max diff over agreeing rays {float(np.max(d[~bad], initial=0.0)):.3e}")
     return nbad

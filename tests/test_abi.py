@@ -33,7 +33,7 @@ def _params(**over):
 def test_invalid_params_rejected(over, msg):
     lib = _lib.load()
     rc = lib.bh_trace_plain_f64(_params(**over), None, None, 0, None, 0, None, None, None, None,
-                                0, None)
+                                None, 0, None)
     assert rc == _lib.BH_EINVAL
     assert msg in lib.bh_last_error().decode()
     with pytest.raises(ValueError):
@@ -44,15 +44,19 @@ def test_invalid_tune_rejected():
     lib = _lib.load()
     for tune in (_lib.TuneParams(96, 1, 1, 0), _lib.TuneParams(512, 1, 1, 0),
                  _lib.TuneParams(256, -1, 1, 0), _lib.TuneParams(256, 1, 65, 0)):
-        rc = lib.bh_trace_plain_f64(_params(), tune, None, 0, None, 0, None, None, None, None, 0,
-                                    None)
+        rc = lib.bh_trace_plain_f64(_params(), tune, None, 0, None, 0, None, None, None, None,
+                                    None, 0, None)
         assert rc == _lib.BH_EINVAL
 
 
 def test_bad_sizes_rejected():
     lib = _lib.load()
-    assert lib.bh_trace_plain_f64(_params(), None, None, 5, None, 5, None, None, None, None, 10,
-                                  None) == _lib.BH_EINVAL
+    assert lib.bh_trace_plain_f64(_params(), None, None, 5, None, 5, None, None, None, None,
+                                  None, 10, None) == _lib.BH_EINVAL
+    assert lib.bh_trace_batched_f64(None, None, 0, 0, None, None, 0, None, 0, None, None, None,
+                                    None, None, None, 0, None) == _lib.BH_EINVAL
+    assert lib.bh_trace_batched_f64(None, None, 300, 0, None, None, 0, None, 0, None, None, None,
+                                    None, None, None, 0, None) == _lib.BH_EINVAL
     assert lib.bh_aos_to_soa_f64(None, None, 3, 4, None) == _lib.BH_EINVAL
     assert lib.bh_geometry_f64(1.0, 2.0, 0, None, 0, None, None, None, None, None, None) \
         == _lib.BH_EINVAL

@@ -1,0 +1,140 @@
+"""GPU: the persistent batched tracer and ray hand-out orders give every ray
+exactly the result trace_rays gives it (bitwise: the same per-ray arithmetic,
+only the schedule differs), and match the reference's golden vectors."""
+
+import numpy as np
+import pytest
+
+from conftest import golden
+from oracle import bh_oracle as orc
+from parity import assert_trace_parity, reference_spread
+from blackhorizon_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _camera_state(job, stride=1):
+    st = orc.Kerr(spin=job.spin)
+    cam = orc.PinholeCamera.from_orbit(job.distance, job.inclination_degrees,
+                                       fov_degrees=job.fov_degrees, width=job.width,
+                                       height=job.height)
+    return orc.camera_state(st, cam)[::stride]
+
+
+def _job_kw(job):
+    return dict(escape_radius=job.escape_radius, max_steps=job.max_steps, rtol=job.rtol,
+                atol=job.atol, capture_margin=job.capture_margin)
+
+
+def _assert_same(a, b, label):
+    np.testing.assert_array_equal(a.status, b.status, err_msg=label)
+    np.testing.assert_array_equal(a.steps, b.steps, err_msg=label)
+    np.testing.assert_array_equal(a.attempts, b.attempts, err_msg=label)
+    np.testing.assert_array_equal(a.states, b.states, err_msg=label)
+    assert a.iterations == b.iterations, label
+
+
+@pytest.mark.parametrize("mode", ["auto", "precise"])
+def test_batch_matches_golden_uneven_subsample(hip, mode):
+    from blackhorizon_amd import batch
+
+    g = golden("uneven_subsample.npz")
+    mix = workloads.uneven_mix()[:8]
+    jobs = [batch.TraceJob(hip.KerrSpacetime(spin=j.spin), g[f"j{k}_state0"], **_job_kw(j))
+            for k, j in enumerate(mix)]
+    res = batch.trace_batch(jobs, mode=mode)
+    for k, (j, r) in enumerate(zip(mix, res)):
+        assert r.iterations == int(g[f"j{k}_iterations"])
+        spread = reference_spread(j.spin, g[f"j{k}_state0"], g[f"j{k}_states"], **_job_kw(j))
+        assert_trace_parity(r.states, r.status, r.steps, g[f"j{k}_states"], g[f"j{k}_status"],
+                            g[f"j{k}_steps"], label=f"batch job {k}", spread=spread)
+        alone = hip.trace_rays(hip.KerrSpacetime(spin=j.spin), g[f"j{k}_state0"], mode=mode,
+                               **_job_kw(j))
+        _assert_same(r, alone, f"job {k}")
+
+
+def test_batch_equals_plain_on_uneven_jobs(hip):
+    """16 jobs of the uneven mix, every 7th camera ray: mixed spins of both
+    signs, distances 20-1000 M, rtol 1e-10..1e-6 (both arithmetic builds in
+    one plan, run on two streams)."""
+    from blackhorizon_amd import batch
+
+    mix = workloads.uneven_mix()[:16]
+    states = [_camera_state(j, stride=7) for j in mix]
+    jobs = [batch.TraceJob(hip.KerrSpacetime(spin=j.spin), s, **_job_kw(j))
+            for j, s in zip(mix, states)]
+    plan = batch.BatchPlan(jobs, mode="auto")
+    assert len(plan.groups) == 2
+    plan.launch()
+    res = plan.results()
+    for k, (j, s, r) in enumerate(zip(mix, states, res)):
+        alone = hip.trace_rays(hip.KerrSpacetime(spin=j.spin), s, **_job_kw(j))
+        _assert_same(r, alone, f"job {k}")
+    # relaunching the same plan reproduces the results
+    plan.launch()
+    res2 = plan.results()
+    for r, r2 in zip(res, res2):
+        _assert_same(r, r2, "relaunch")
+
+
+def test_batch_edge_jobs(hip):
+    """Empty jobs, a job with max_steps 0, a single-ray job and tuning
+    shapes; ValueError when one job starts outside its escape radius."""
+    from blackhorizon_amd import batch
+
+    g = golden("edge_cases.npz")
+    st = hip.KerrSpacetime(spin=0.9)
+    s0 = g["default_state0"]
+    jobs = [batch.TraceJob(st, np.zeros((0, 8)), 40.0),
+            batch.TraceJob(st, s0, 40.0),
+            batch.TraceJob(st, s0, 40.0, max_steps=0),
+            batch.TraceJob(st, np.zeros((0, 8)), 40.0),
+            batch.TraceJob(st, s0[1:2], 40.0, rtol=1e-6),
+            batch.TraceJob(hip.KerrSpacetime(spin=-0.3), s0, 40.0, rtol=1e-8)]
+    for tune in (None, hip.Tune(block_size=64, refill_min=1), hip.Tune(refill_min=16)):
+        res = batch.trace_batch(jobs, tune=tune)
+        assert res[0].states.shape == (0, 8) and res[0].iterations == 0
+        assert res[3].states.shape == (0, 8)
+        for k in (1, 2, 4, 5):
+            j = jobs[k]
+            alone = hip.trace_rays(j.spacetime, j.state0, j.escape_radius, max_steps=j.max_steps,
+                                   rtol=j.rtol)
+            _assert_same(res[k], alone, f"edge job {k}")
+        assert np.all(res[2].status == 3) and res[2].iterations == 0
+    bad = [batch.TraceJob(st, s0, 40.0), batch.TraceJob(st, s0, 15.0)]
+    with pytest.raises(ValueError):
+        batch.trace_batch(bad)
+
+
+def test_batch_order_does_not_change_results(hip):
+    from blackhorizon_amd import batch
+
+    mix = workloads.uneven_mix()[:6]
+    states = [_camera_state(j, stride=11) for j in mix]
+    jobs = [batch.TraceJob(hip.KerrSpacetime(spin=j.spin), s, **_job_kw(j))
+            for j, s in zip(mix, states)]
+
+    def reverse(jobs_in_group, offsets):
+        return np.arange(int(offsets[-1]))[::-1].copy()
+
+    a = batch.trace_batch(jobs, mode="precise")
+    b = batch.trace_batch(jobs, mode="precise", order_fn=reverse)
+    for r, r2 in zip(a, b):
+        _assert_same(r, r2, "reversed queue")
+
+
+@pytest.mark.parametrize("tile", [4, 8, 16])
+def test_tile_order_bitwise_identical(hip, tile):
+    job = workloads.small_frame()
+    cam = hip.PinholeCamera.from_orbit(job.distance, job.inclination_degrees,
+                                       fov_degrees=job.fov_degrees, width=job.width,
+                                       height=job.height)
+    order = cam.tile_order(tile)
+    assert np.array_equal(np.sort(order), np.arange(job.n_rays))
+    s0 = _camera_state(job)
+    st = hip.KerrSpacetime(spin=job.spin)
+    a = hip.trace_rays(st, s0, **_job_kw(job))
+    b = hip.trace_rays(st, s0, ray_order=order, **_job_kw(job))
+    _assert_same(a, b, f"tile {tile}")
+    with pytest.raises(ValueError):
+        hip.trace_rays(st, s0, ray_order=np.zeros(job.n_rays, dtype=np.int64), **_job_kw(job))

@@ -5,12 +5,14 @@ Bars: precise build -- geometry, RHS, DP5 stages and initial conditions equal
 the oracle bit for bit (every operation correctly rounded, NumPy order);
 traces -- identical status, accepted steps and iterations, final states
 within 1e-7 relative (t, position, p_t and momentum blocks, each relative to
-max(norm, 1)).  The precise build meets it at every tolerance.  The fast
-build rounds the RHS differently in the last ulp; the embedded error
-estimate amplifies that by ~1/rtol and the step sizes follow it, so its
-final states meet 1e-7 at rtol >= 1e-9 (the bar is checked there) and drift
-to ~1e-6 at rtol 1e-10 (bounded separately).  mode="auto" (the default)
-picks fast at rtol >= 1e-9 and precise below.
+max(norm, 1)); on rays where the reference itself moves by more than that
+between hosts (NumPy's SIMD pow vs libm pow: captured rays ending inside the
+horizon at tight rtol), within 10x that host-to-host spread.  The precise
+build meets this at every tolerance.  The fast build rounds the RHS
+differently in the last ulp; the embedded error estimate amplifies that by
+~1/rtol and the step sizes follow it, so it is held to the bar at
+rtol >= 1e-8 and bounded separately below.  mode="auto" (the default) picks
+fast at rtol >= 1e-8 and precise below.
 """
 
 import math
@@ -21,14 +23,14 @@ import pytest
 
 from conftest import golden
 from oracle import bh_oracle as orc
-from parity import assert_trace_parity, max_rel_diff
+from parity import assert_trace_parity, max_rel_diff, reference_spread
 from blackhorizon_amd import workloads
 
 pytestmark = pytest.mark.gpu
 
 SPINS = [0.0, 0.5, 0.9, -0.998]
 MODES = ["precise", "fast", "auto"]
-FAST_MIN_RTOL = 1e-9
+FAST_MIN_RTOL = 1e-8
 
 
 def mode_applies(mode, rtol):
@@ -151,17 +153,28 @@ def _kw(g, name):
     return kw
 
 
+_SPREADS = {}
+
+
+def _spread(key, spin, state0, ref_states, kw):
+    """Cached host-to-host spread of the reference's final states."""
+    if key not in _SPREADS:
+        _SPREADS[key] = reference_spread(spin, state0, ref_states, **kw)
+    return _SPREADS[key]
+
+
 @pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("case", EDGE_CASES)
 def test_trace_edge_cases(hip, case, mode):
     g = golden("edge_cases.npz")
     kw = _kw(g, case)
     if not mode_applies(mode, kw.get("rtol", 1e-9)):
-        pytest.skip("fast build at rtol < 1e-9: see test_fast_build_tight_tolerance_bounded")
+        pytest.skip("fast build at rtol < 1e-8: see test_fast_build_tight_tolerance_bounded")
     res = hip.trace_rays(hip.KerrSpacetime(spin=0.9), g[case + "_state0"], mode=mode, **kw)
     assert res.iterations == int(g[case + "_iterations"])
+    spread = _spread(("edge", case), 0.9, g[case + "_state0"], g[case + "_states"], kw)
     assert_trace_parity(res.states, res.status, res.steps, g[case + "_states"],
-                        g[case + "_status"], g[case + "_steps"], label=case)
+                        g[case + "_status"], g[case + "_steps"], label=case, spread=spread)
     assert np.all(res.attempts >= res.steps)
     assert int(np.max(res.attempts, initial=0)) == res.iterations
 
@@ -186,8 +199,11 @@ def test_trace_golden_small_frame(hip, mode):
                          max_steps=job.max_steps, rtol=job.rtol, atol=job.atol,
                          capture_margin=job.capture_margin, mode=mode)
     assert res.iterations == int(g["iterations"])
+    kw = dict(escape_radius=job.escape_radius, max_steps=job.max_steps, rtol=job.rtol,
+              atol=job.atol, capture_margin=job.capture_margin)
+    spread = _spread("small8", job.spin, g["state0"], g["states"], kw)
     assert_trace_parity(res.states, res.status, res.steps, g["states"], g["status"], g["steps"],
-                        label="small frame")
+                        label="small frame", spread=spread)
 
 
 @pytest.mark.parametrize("mode", MODES)
@@ -200,8 +216,10 @@ def test_trace_golden_fans_uneven_hd(hip, mode):
         res = hip.trace_rays(hip.KerrSpacetime(spin=spin), g[k + "_state0"], 60.0, rtol=1e-10,
                              mode=mode)
         assert res.iterations == int(g[k + "_iterations"])
+        spread = _spread(("fan", k), spin, g[k + "_state0"], g[k + "_states"],
+                         dict(escape_radius=60.0, rtol=1e-10))
         assert_trace_parity(res.states, res.status, res.steps, g[k + "_states"],
-                            g[k + "_status"], g[k + "_steps"], label=k)
+                            g[k + "_status"], g[k + "_steps"], label=k, spread=spread)
     g = golden("uneven_subsample.npz")
     for j, job in enumerate(workloads.uneven_mix()[:8]):
         if not mode_applies(mode, job.rtol):
@@ -210,15 +228,22 @@ def test_trace_golden_fans_uneven_hd(hip, mode):
                              job.escape_radius, max_steps=job.max_steps, rtol=job.rtol,
                              atol=job.atol, capture_margin=job.capture_margin, mode=mode)
         assert res.iterations == int(g[f"j{j}_iterations"])
+        kw = dict(escape_radius=job.escape_radius, max_steps=job.max_steps, rtol=job.rtol,
+                  atol=job.atol, capture_margin=job.capture_margin)
+        spread = _spread(("job", j), job.spin, g[f"j{j}_state0"], g[f"j{j}_states"], kw)
         assert_trace_parity(res.states, res.status, res.steps, g[f"j{j}_states"],
-                            g[f"j{j}_status"], g[f"j{j}_steps"], label=f"job {j}")
+                            g[f"j{j}_status"], g[f"j{j}_steps"], label=f"job {j}", spread=spread)
     g = golden("hd_subsample.npz")
     for f, job in enumerate(workloads.hd_frames()):
         res = hip.trace_rays(hip.KerrSpacetime(spin=job.spin), g[f"f{f}_state0"],
                              job.escape_radius, max_steps=job.max_steps, rtol=job.rtol,
                              atol=job.atol, capture_margin=job.capture_margin, mode=mode)
+        kw = dict(escape_radius=job.escape_radius, max_steps=job.max_steps, rtol=job.rtol,
+                  atol=job.atol, capture_margin=job.capture_margin)
+        spread = _spread(("hd", f), job.spin, g[f"f{f}_state0"], g[f"f{f}_states"], kw)
         assert_trace_parity(res.states, res.status, res.steps, g[f"f{f}_states"],
-                            g[f"f{f}_status"], g[f"f{f}_steps"], label=f"hd frame {f}")
+                            g[f"f{f}_status"], g[f"f{f}_steps"], label=f"hd frame {f}",
+                            spread=spread)
 
 
 @pytest.fixture(scope="module")
@@ -229,23 +254,24 @@ def small_frame_oracle():
                                        fov_degrees=job.fov_degrees, width=job.width,
                                        height=job.height)
     s0 = orc.camera_state(st, cam)
-    res, att = orc.trace_rays(st, s0, job.escape_radius, max_steps=job.max_steps, rtol=job.rtol,
-                              atol=job.atol, capture_margin=job.capture_margin,
-                              return_attempts=True)
-    return job, s0, res, att
+    kw = dict(escape_radius=job.escape_radius, max_steps=job.max_steps, rtol=job.rtol,
+              atol=job.atol, capture_margin=job.capture_margin)
+    res, att = orc.trace_rays(st, s0, return_attempts=True, **kw)
+    spread = reference_spread(job.spin, s0, res.states, **kw)
+    return job, s0, res, att, spread
 
 
 @pytest.mark.parametrize("mode", MODES)
 def test_trace_full_small_frame_vs_oracle(hip, small_frame_oracle, mode):
     """All 19,200 rays of the 160x120 a = 0.9 frame against the oracle,
     including the E, L_z and Hamiltonian drift bound."""
-    job, s0, ref, ref_att = small_frame_oracle
+    job, s0, ref, ref_att, spread = small_frame_oracle
     st = hip.KerrSpacetime(spin=job.spin)
     res = hip.trace_rays(st, s0, job.escape_radius, max_steps=job.max_steps, rtol=job.rtol,
                          atol=job.atol, capture_margin=job.capture_margin, mode=mode)
     assert res.iterations == ref.iterations
     assert_trace_parity(res.states, res.status, res.steps, ref.states, ref.status, ref.steps,
-                        label=f"small frame {mode}")
+                        label=f"small frame {mode}", spread=spread)
     np.testing.assert_array_equal(res.attempts, ref_att)
     # drift no worse than the reference's: E exact, L_z and H_ham within the
     # oracle's drift plus round-off

@@ -145,3 +145,33 @@ def test_oracle_shadow_png_subsample():
     light = orc.checker_parity(st, res.states[esc])
     agree = np.mean(light == (want[esc] == 2))
     assert agree > 0.999, agree
+
+
+def test_reference_spread_small_on_escaped_rays():
+    """The host-to-host spread of the reference (NumPy SIMD pow vs libm pow)
+    is tiny on escaped rays and only exceeds 1e-7 on captured rays, which end
+    inside the horizon with momenta of order 1e3."""
+    from parity import reference_spread
+
+    g = golden("small_frame_stride8.npz")
+    job = workloads.small_frame()
+    spread = reference_spread(job.spin, g["state0"], g["states"],
+                              escape_radius=job.escape_radius, rtol=job.rtol, atol=job.atol,
+                              capture_margin=job.capture_margin)
+    esc = g["status"] == 2
+    assert spread[esc].max() < 1e-8
+    assert spread.max() < 1e-6
+    assert np.linalg.norm(g["states"][~esc, 5:8], axis=-1).min() > 50.0
+
+
+def test_camera_tile_order_is_permutation():
+    from blackhorizon_amd.camera import PinholeCamera
+
+    for w, h, t in ((160, 120, 8), (13, 7, 4), (1, 1, 8), (64, 3, 16)):
+        cam = PinholeCamera.from_orbit(30.0, 60.0, width=w, height=h)
+        o = cam.tile_order(t)
+        assert np.array_equal(np.sort(o), np.arange(w * h))
+        if w == 160:
+            first = o[:64]
+            assert np.array_equal(np.unique(first // 160), np.arange(8))
+            assert np.array_equal(np.unique(first % 160), np.arange(8))
