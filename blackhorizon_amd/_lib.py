@@ -70,6 +70,12 @@ SIGNATURES = {
                            _P, _I64, _P, _P, _P, _P, _P, _I64, _P],
     "bh_trace_batched_f64": [_P, _P, _I32, _I32, ctypes.POINTER(TuneParams), _P, _I64, _P, _I64,
                              _P, _P, _P, _P, _P, _P, _I64, _P],
+    "bh_adjoint_scratch_doubles": [_I32],
+    "bh_trace_record_f64": [ctypes.POINTER(TraceParams), _P, _I64, _P, _I64, _P, _P, _P, _P, _I64,
+                            _I32, _I32, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "bh_trace_adjoint_f64": [ctypes.POINTER(TraceParams), _I64, _I32, _D, _P, _P, _P, _P, _P, _P,
+                             _P, _P, _P, _P, _P],
+    "bh_rhs_vjp_f64": [_D, _D, _P, _P, _I64, _P, _P, _P],
     "bh_aos_to_soa_f64": [_P, _P, _I64, _I64, _P],
     "bh_soa_to_aos_f64": [_P, _I64, _P, _I64, _P],
     "bh_geometry_f64": [_D, _D, _I32, _P, _I64, _P, _P, _P, _P, _P, _P],

@@ -181,6 +181,66 @@ int32_t bh_trace_batched_f64(const bh_trace_params *jobs, const int64_t *ray_off
   return hip_status(e, "bh_trace_batched_f64");
 }
 
+int32_t bh_adjoint_scratch_doubles(int32_t K) { return K < 1 ? -1 : (int32_t)adjoint_scratch_per_ray(K); }
+
+int32_t bh_trace_record_f64(const bh_trace_params *p, const double *state_in_soa, int64_t ld_in,
+                            double *state_out_soa, int64_t ld_out, int32_t *status,
+                            int64_t *steps, int64_t *attempts, uint64_t *iterations, int64_t n,
+                            int32_t K, int32_t cap, double score_radius, double *ckpt,
+                            int32_t *n_ckpt, double *term, double *prev, double *evy1,
+                            int64_t *ev_step, double *theta, int32_t *event, void *stream) {
+  int32_t rc = check_params(p);
+  if (rc) return rc;
+  if (n < 0 || ld_in < n || ld_out < n) return fail(BH_EINVAL, "bad n / ld");
+  if (K < 1 || cap < 1) return fail(BH_EINVAL, "K and cap must be >= 1");
+  if (n > 0 && (!state_in_soa || !state_out_soa || !status || !steps || !attempts || !ckpt ||
+                !n_ckpt || !term || !prev || !evy1 || !ev_step || !theta || !event))
+    return fail(BH_EINVAL, "NULL buffer");
+  hipStream_t s = (hipStream_t)stream;
+  JobConst c = make_job_const(*p);
+  TraceIO io{state_in_soa, ld_in, state_out_soa, ld_out, status, steps, attempts,
+             (unsigned long long *)iterations, nullptr, n};
+  RecordIO rec{ckpt, n_ckpt, term, prev, evy1, ev_step, theta, event, score_radius, cap, K};
+  hipError_t e = hipSuccess;
+  if (iterations) e = launch_zero_u64((unsigned long long *)iterations, s);
+  if (e == hipSuccess)
+    e = p->mode == BH_MODE_PRECISE ? precise::launch_trace_record(c, io, rec, s)
+                                   : fast::launch_trace_record(c, io, rec, s);
+  return hip_status(e, "bh_trace_record_f64");
+}
+
+int32_t bh_trace_adjoint_f64(const bh_trace_params *p, int64_t n, int32_t K, double score_radius,
+                             const double *ckpt, const int32_t *event, const int64_t *ev_step,
+                             const double *evy1, const double *prev, const double *theta,
+                             const double *gterm, double *gy0, double *gspin, double *scratch,
+                             void *stream) {
+  int32_t rc = check_params(p);
+  if (rc) return rc;
+  if (n < 0) return fail(BH_EINVAL, "bad n");
+  if (K < 1) return fail(BH_EINVAL, "K must be >= 1");
+  if (n > 0 && (!ckpt || !event || !ev_step || !evy1 || !prev || !theta || !gterm || !gy0 ||
+                !gspin || !scratch))
+    return fail(BH_EINVAL, "NULL buffer");
+  JobConst c = make_job_const(*p);
+  AdjointIO a{ckpt, event, ev_step, evy1, prev, theta, gterm, gy0, gspin, scratch, score_radius,
+              n, K};
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = p->mode == BH_MODE_PRECISE ? precise::launch_trace_adjoint(c, a, s)
+                                            : fast::launch_trace_adjoint(c, a, s);
+  return hip_status(e, "bh_trace_adjoint_f64");
+}
+
+int32_t bh_rhs_vjp_f64(double mass, double spin, const double *state, const double *v,
+                       int64_t n, double *zbar, double *abar, void *stream) {
+  int32_t rc = check_spacetime(mass, spin);
+  if (rc) return rc;
+  if (n < 0) return fail(BH_EINVAL, "bad n");
+  if (n > 0 && (!state || !v || !zbar || !abar)) return fail(BH_EINVAL, "NULL buffer");
+  JobConst c = spacetime_const(mass, spin);
+  return hip_status(fast::launch_rhs_vjp(c, state, v, n, zbar, abar, (hipStream_t)stream),
+                    "bh_rhs_vjp_f64");
+}
+
 int32_t bh_aos_to_soa_f64(const double *aos, double *soa, int64_t ld, int64_t n, void *stream) {
   if (n < 0 || ld < n) return fail(BH_EINVAL, "bad n / ld");
   if (n > 0 && (!aos || !soa)) return fail(BH_EINVAL, "NULL buffer");

@@ -58,6 +58,41 @@ struct TraceIO {
   int64_t n;
 };
 
+// Forward-with-checkpoints outputs (SoA, plane stride n).
+struct RecordIO {
+  double *ckpt;      // [cap][9][n]: state (8) and controller step size every K accepted steps
+  int32_t *n_ckpt;   // checkpoints written per ray
+  double *term;      // [8][n]: terminal point
+  double *prev;      // [9][n]: start state and step size of the event step
+  double *evy1;      // [8][n]: state at the end of the event step
+  int64_t *ev_step;  // [n]: accepted-step index at the end of the event step
+  double *theta;     // [n]: dense-output parameter of the event in that step
+  int32_t *event;    // [n]: 0 none, 1 escape sphere, 2 scoring sphere, -1 checkpoint overflow
+  double score_radius;
+  int32_t cap;
+  int32_t K;
+};
+
+struct AdjointIO {
+  const double *ckpt;
+  const int32_t *event;
+  const int64_t *ev_step;
+  const double *evy1;
+  const double *prev;
+  const double *theta;
+  const double *gterm;  // [8][n] adjoint of the terminal point
+  double *gy0;          // [8][n] adjoint of the initial state
+  double *gspin;        // [n] per-ray spin adjoint
+  double *scratch;      // [9 K + 40][n]
+  double score_radius;
+  int64_t n;
+  int32_t K;
+};
+
+// Doubles of adjoint scratch per ray: K replayed states + step sizes and
+// five parked DP5 stages.
+constexpr int64_t adjoint_scratch_per_ray(int64_t K) { return 9 * K + 40; }
+
 // Jobs one batched launch can hold (job table and ray offsets live in LDS).
 constexpr int MAX_BATCH_JOBS = 256;
 
@@ -76,6 +111,12 @@ struct LaunchShape {
                                   int n_jobs, const TraceIO &io,                   \
                                   unsigned long long *queue, const LaunchShape &shape, \
                                   hipStream_t s);                                  \
+  hipError_t launch_trace_record(const JobConst &c, const TraceIO &io,            \
+                                 const RecordIO &rec, hipStream_t s);              \
+  hipError_t launch_trace_adjoint(const JobConst &c, const AdjointIO &a,           \
+                                  hipStream_t s);                                  \
+  hipError_t launch_rhs_vjp(const JobConst &c, const double *state, const double *v, \
+                            int64_t n, double *zbar, double *abar, hipStream_t s); \
   hipError_t launch_count_outside(const JobConst &c, const double *in, int64_t ld, \
                                   int64_t n, int32_t *count, hipStream_t s);       \
   hipError_t launch_geometry(const JobConst &c, const double *xyz, int64_t n,      \

@@ -129,6 +129,50 @@ int32_t bh_trace_batched_f64(const bh_trace_params *jobs, const int64_t *ray_off
                              const int64_t *order, uint64_t *workspace, int64_t n_total,
                              void *stream);
 
+/* ---- discrete adjoint (differentiable tracing) -------------------------
+ * No counterpart in the reference (it has no derivatives); these extend
+ * trace_rays (tracer.py:50-157) to gradients of each ray's terminal point
+ * with respect to its initial state and the spin. */
+
+/* Doubles of adjoint scratch per ray for checkpoint interval K (9 K + 40),
+ * or -1 for K < 1. */
+int32_t bh_adjoint_scratch_doubles(int32_t K);
+
+/* Forward trace exactly as bh_trace_plain_f64 (same statuses, steps and
+ * final states; one ray per lane), recording for the adjoint:
+ *  ckpt  [cap][9][n]: state and controller step size every K accepted steps;
+ *  n_ckpt [n]; term [8][n]: terminal point -- escaped rays: where the cubic
+ *  Hermite dense output of the last step crosses the escape sphere;
+ *  captured rays: where the step that first enters the sphere
+ *  r = score_radius crosses it; otherwise the final state;
+ *  prev [9][n] / evy1 [8][n]: start state + step size / end state of that
+ *  event step; ev_step [n]: its accepted-step index; theta [n]: its dense-
+ *  output parameter; event [n]: 0 none, 1 escape, 2 scoring sphere,
+ *  -1 checkpoint capacity exceeded (no gradient). */
+int32_t bh_trace_record_f64(const bh_trace_params *p, const double *state_in_soa, int64_t ld_in,
+                            double *state_out_soa, int64_t ld_out, int32_t *status,
+                            int64_t *steps, int64_t *attempts, uint64_t *iterations, int64_t n,
+                            int32_t K, int32_t cap, double score_radius, double *ckpt,
+                            int32_t *n_ckpt, double *term, double *prev, double *evy1,
+                            int64_t *ev_step, double *theta, int32_t *event, void *stream);
+
+/* Reverse pass: given gterm [8][n] = dL/d(terminal point), writes
+ * gy0 [8][n] = dL/d(initial state) and gspin [n] = per-ray dL/d(spin),
+ * differentiating every accepted step with its step size held fixed (each
+ * checkpoint segment is replayed through the controller).  scratch:
+ * [bh_adjoint_scratch_doubles(K)][n] device doubles.  Rays with event -1
+ * get NaN. */
+int32_t bh_trace_adjoint_f64(const bh_trace_params *p, int64_t n, int32_t K, double score_radius,
+                             const double *ckpt, const int32_t *event, const int64_t *ev_step,
+                             const double *evy1, const double *prev, const double *theta,
+                             const double *gterm, double *gy0, double *gspin, double *scratch,
+                             void *stream);
+
+/* Vector-Jacobian product of geodesic_rhs (geodesics.py:42-74) at n states
+ * (rows): zbar = J^T v (n, 8) and abar = v . d f / d spin (n). */
+int32_t bh_rhs_vjp_f64(double mass, double spin, const double *state, const double *v,
+                       int64_t n, double *zbar, double *abar, void *stream);
+
 /* (n, 8) row-major <-> 8 planes of ld.  Replaces nothing in the reference
  * (its arrays stay (n, 8)); the boundary transpose of the SoA design. */
 int32_t bh_aos_to_soa_f64(const double *aos, double *soa, int64_t ld, int64_t n,

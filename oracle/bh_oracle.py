@@ -363,6 +363,23 @@ def step_factor_libm(ratio):
     return np.clip(0.9 * p, 0.2, 5.0)
 
 
+def step_factor_ulp_noise(seed: int):
+    """step_factor whose pow result is moved by -1, 0 or +1 ulp at random:
+    a stand-in for any faithfully rounded pow.  Tracing with it measures how
+    far an ulp-level difference in the step factors moves each ray's final
+    state (see step_factor_libm)."""
+    rng = np.random.default_rng(seed)
+
+    def fn(ratio):
+        safe = np.maximum(ratio, 1e-16)
+        p = safe ** (-0.2)
+        p = np.where(rng.integers(-1, 2, size=np.shape(p)) > 0, np.nextafter(p, np.inf),
+                     np.where(rng.integers(0, 2, size=np.shape(p)) > 0, np.nextafter(p, 0.0), p))
+        return np.clip(0.9 * p, 0.2, 5.0)
+
+    return fn
+
+
 def integrate_dense(rhs, state0, lambda_total, rtol=1e-10, atol=1e-12,
                     initial_step=0.05, max_step=2.0):
     """Batch-shared adaptive DP5 over an affine interval; integrators.py:127-174."""

@@ -58,17 +58,22 @@ def block_diff(got, ref):
 
 
 def reference_spread(spin, state0, ref_states, **trace_kw):
-    """How far the reference's own final states move when NumPy's power ufunc
-    is replaced by libm pow (a different but equally valid host): traced by
-    the oracle, which reproduces the reference bit for bit on this host."""
+    """How far the reference's own final states move under ulp-level changes
+    of its step factors: the largest block difference over three re-traces
+    by the oracle (which reproduces the reference bit for bit on this host),
+    one with libm pow in place of NumPy's power ufunc (a different but
+    equally valid host) and two with the pow result moved by a random ulp."""
     from oracle import bh_oracle as orc
     import warnings
 
-    with warnings.catch_warnings():
-        warnings.simplefilter("ignore", RuntimeWarning)
-        alt = orc.trace_rays(orc.Kerr(spin=spin), state0, factor_fn=orc.step_factor_libm,
-                             **trace_kw)
-    return block_diff(alt.states, ref_states)
+    worst = np.zeros(np.asarray(state0).shape[0])
+    for fn in (orc.step_factor_libm, orc.step_factor_ulp_noise(1),
+               orc.step_factor_ulp_noise(2)):
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore", RuntimeWarning)
+            alt = orc.trace_rays(orc.Kerr(spin=spin), state0, factor_fn=fn, **trace_kw)
+        worst = np.maximum(worst, block_diff(alt.states, ref_states))
+    return worst
 
 
 SPREAD_FACTOR = 10.0
