@@ -259,12 +259,76 @@ def run_uneven(args) -> dict:
         jobs_per_mode=modes, per_ray_steps_identical=bool(same and steps_b == steps_p))
 
 
+def run_shadowfit(args) -> dict:
+    """512 x 512 differentiable shadow fit at a = 0.7, inclination 45 deg,
+    r_obs = 40 M, FOV 30 deg, escape sphere 45 M, rtol 1e-8, checkpoints
+    every 32 accepted steps; target traced at a = 0.75, inclination 50 deg.
+    Times the plain forward trace, the recording forward and the adjoint."""
+    import math
+
+    import torch
+
+    from blackhorizon_amd import _lib, adjoint, tracer
+
+    _lib.require_device()
+    size = args.fit_size
+    r_plus = lambda a: 1.0 + math.sqrt(1.0 - a * a)  # noqa: E731
+    cfg = adjoint.TraceSettings(escape_radius=45.0, rtol=1e-8, atol=1e-10,
+                                capture_margin=0.01 / r_plus(0.7), K=32, mode=args.mode)
+    cfg_t = adjoint.TraceSettings(escape_radius=45.0, rtol=1e-8, atol=1e-10,
+                                  capture_margin=0.01 / r_plus(0.75), K=32, mode=args.mode)
+    with torch.no_grad():
+        tgt = adjoint.camera_state(0.75, 50.0, 40.0, 30.0, size, size)
+        Yt, st_t = adjoint.trace_terminal(tgt, 0.75, cfg_t)
+        dirs_t = adjoint.terminal_directions(Yt)
+    p = [torch.tensor(v, dtype=torch.float64, device="cuda", requires_grad=True)
+         for v in (0.7, 45.0, 40.0)]
+    y0 = adjoint.camera_state(p[0], p[1], p[2], 30.0, size, size)
+    cur = torch.cuda.current_stream()
+    # plain forward on the same rays (the reference-equivalent trace)
+    st = tracer.KerrSpacetime(1.0, 0.7)
+
+    def ev():
+        return torch.cuda.Event(enable_timing=True)
+
+    times = {"plain": [], "record": [], "adjoint": []}
+    for k in range(args.warmup + args.steps):
+        e = [ev() for _ in range(4)]
+        torch.cuda.synchronize()
+        e[0].record(cur)
+        res = tracer.trace_rays(st, y0.detach(), 45.0, rtol=1e-8, atol=1e-10,
+                                capture_margin=cfg.capture_margin, mode=args.mode)
+        e[1].record(cur)
+        Y, status = adjoint.trace_terminal(y0, p[0], cfg)
+        loss = adjoint.shadow_loss(Y, status, dirs_t, st_t)
+        e[2].record(cur)
+        g = torch.autograd.grad(loss, p, retain_graph=True)
+        e[3].record(cur)
+        torch.cuda.synchronize()
+        if k >= args.warmup:
+            times["plain"].append(e[0].elapsed_time(e[1]))
+            times["record"].append(e[1].elapsed_time(e[2]))
+            times["adjoint"].append(e[2].elapsed_time(e[3]))
+        print(f"[bench] shadowfit pass {k} done", file=sys.stderr, flush=True)
+    med = {k: float(np.median(v)) for k, v in times.items()}
+    steps = int(res.steps.sum().item())
+    return dict(rays=size * size, ray_steps=steps, loss=float(loss),
+                grad_spin=float(g[0]), grad_inclination=float(g[1]), grad_distance=float(g[2]),
+                ms_plain_forward=med["plain"], ms_recording_forward=med["record"],
+                ms_adjoint=med["adjoint"],
+                adjoint_over_forward=med["adjoint"] / med["plain"],
+                forward_plus_adjoint_over_forward=(med["record"] + med["adjoint"]) / med["plain"],
+                status_counts=torch.bincount(status.long(), minlength=7).tolist())
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="hd", choices=["hd", "hd09", "small", "uneven"])
+    ap.add_argument("--workload", default="hd",
+                    choices=["hd", "hd09", "small", "uneven", "shadowfit"])
+    ap.add_argument("--fit-size", type=int, default=512, help="shadowfit: image side")
     ap.add_argument("--serial-groups", action="store_true",
                     help="uneven: run the fast and precise batches one after the other")
     ap.add_argument("--mode", default="auto", choices=["auto", "fast", "precise"])
@@ -284,6 +348,11 @@ def main(argv=None) -> int:
                                    "measured; run with --gpus 1"}))
         return 2
 
+    if args.workload == "shadowfit":
+        res = run_shadowfit(args)
+        print(json.dumps({"metric": METRIC, "workload": "shadow fit", "mode": args.mode,
+                          **res}), flush=True)
+        return 0
     if args.workload == "uneven":
         res = run_uneven(args)
         print(json.dumps({"metric": METRIC, "workload": "uneven 64-job mix", "mode": args.mode,

@@ -230,16 +230,21 @@ def camera_state(spin, inclination_degrees, distance, fov_degrees: float, width:
 # ---------------------------------------------------------------------------
 
 def shadow_loss(terminal: torch.Tensor, status: torch.Tensor, target_dirs: torch.Tensor,
-                target_status: torch.Tensor, mismatch_cost: float = 1.0) -> torch.Tensor:
+                target_status: torch.Tensor, mismatch_cost: float = 1.0,
+                classes: tuple = (1, 2)) -> torch.Tensor:
     """Sum over rays of |n - n_target|^2 with n the unit position of the
     terminal point (the squared chord between the two directions, the smooth
     form of the squared angular distance), for rays whose class matches the
     target's: escaped rays by where they leave through the escape sphere,
     captured rays by where they plunge through the scoring sphere near the
-    horizon; rays whose class differs cost ``mismatch_cost`` each."""
+    horizon; rays whose class differs cost ``mismatch_cost`` each.
+    ``classes`` limits the scored classes (1 captured, 2 escaped)."""
     pos = terminal[:, 1:4]
     n = pos / torch.linalg.norm(pos, dim=-1, keepdim=True)
-    same = (status == target_status) & ((status == 1) | (status == 2))
+    scored = torch.zeros_like(status, dtype=torch.bool)
+    for c in classes:
+        scored |= status == c
+    same = (status == target_status) & scored
     per_ray = ((n - target_dirs) ** 2).sum(-1)
     return torch.where(same, per_ray, torch.full_like(per_ray, mismatch_cost)).sum()
 

@@ -142,7 +142,7 @@ def _hermite(th, h, y0, f0, y1, f1, deriv=False):
 
 def trace_terminal(state0, spin, escape_radius, max_steps=20000, rtol=1e-9, atol=1e-12,
                    initial_step=0.1, max_step=2.0, min_step=1e-10, capture_margin=1e-3,
-                   mass=1.0, score_factor=2.0):
+                   mass=1.0, score_factor=3.0, return_steps=False):
     """Terminal points (n, 8) and statuses of a trace, differentiable in
     ``state0`` and ``spin`` through the accepted steps (fixed step sizes).
 
@@ -167,6 +167,7 @@ def trace_terminal(state0, spin, escape_radius, max_steps=20000, rtol=1e-9, atol
     crossed = torch.zeros(n, dtype=torch.bool)
     ys0, ys1, hs = state0, state0, torch.zeros(n, dtype=torch.float64)
     r_cur = radius(a_val, state0.detach()[:, 1:4])
+    h_hist = []
     for _ in range(4 * max_steps):
         active = status == 0
         if not bool(active.any()):
@@ -177,6 +178,7 @@ def trace_terminal(state0, spin, escape_radius, max_steps=20000, rtol=1e-9, atol
         accept = (ratio <= 1.0) & active
         y_prev = torch.where(accept[:, None], y, y_prev)
         h_prev = torch.where(accept, h, h_prev)
+        h_hist.append(torch.where(accept, h, torch.full_like(h, float("nan"))))
         y = torch.where(accept[:, None], new, y)
         steps = steps + accept.to(torch.int64)
         h = torch.where(active, torch.clamp(h * step_factor(ratio), 0.0, max_step), h)
@@ -201,8 +203,12 @@ def trace_terminal(state0, spin, escape_radius, max_steps=20000, rtol=1e-9, atol
     esc = (status == 2) & (steps > 0)
     cap = (status == 1) & crossed
     ev = esc | cap
+    extra = ()
+    if return_steps:
+        hh = torch.stack(h_hist, 1) if h_hist else torch.zeros((n, 0), dtype=torch.float64)
+        extra = ([hh[i][~torch.isnan(hh[i])] for i in range(n)],)
     if not bool(ev.any()):
-        return y, status, steps
+        return (y, status, steps) + extra
     idx = torch.nonzero(ev)[:, 0]
     is_esc = esc[idx]
     y0 = torch.where(is_esc[:, None], y_prev[idx], ys0[idx])
@@ -210,13 +216,21 @@ def trace_terminal(state0, spin, escape_radius, max_steps=20000, rtol=1e-9, atol
     hp = torch.where(is_esc, h_prev[idx], hs[idx])
     r_plus_t = mass + torch.sqrt(mass * mass - spin * spin)
     Rr = torch.where(is_esc, torch.full_like(hp, float(escape_radius)), r_plus_t * score_factor)
+    term = _event_point(mass, spin, a_val, y0, y1, hp, Rr)
+    out = y.clone()
+    out = out.index_put((idx,), term)
+    return (out, status, steps) + extra
+
+
+def _event_point(mass, spin, a_val, y0, y1, hp, Rr):
+    """Point where the Hermite dense output of the steps y0 -> y1 (sizes hp)
+    crosses r = Rr, differentiable with the implicit derivative of theta."""
     f0, f1 = rhs(mass, spin, y0), rhs(mass, spin, y1)
-    # bisection on detached values
     with torch.no_grad():
         Rd = Rr.detach()
         g = lambda t: radius(a_val, _hermite(t, hp, y0, f0, y1, f1)[:, 1:4]) - Rd  # noqa: E731
-        lo = torch.zeros(len(idx), dtype=torch.float64)
-        hi = torch.ones(len(idx), dtype=torch.float64)
+        lo = torch.zeros(y0.shape[0], dtype=torch.float64)
+        hi = torch.ones(y0.shape[0], dtype=torch.float64)
         sgn0 = torch.sign(g(lo))
         for _ in range(80):
             mid = 0.5 * (lo + hi)
@@ -233,7 +247,42 @@ def trace_terminal(state0, spin, escape_radius, max_steps=20000, rtol=1e-9, atol
         gr = torch.autograd.grad(radius(a_val, pos).sum(), pos)[0]
     gprime = (gr * dYt[:, 1:4].detach()).sum(-1)
     th_ad = th - (rr - Rr) / gprime
-    term = _hermite(th_ad, hp, y0, f0, y1, f1)
-    out = y.clone()
-    out = out.index_put((idx,), term)
-    return out, status, steps
+    return _hermite(th_ad, hp, y0, f0, y1, f1)
+
+
+def terminal_fixed_steps(state0, spin, h_lists, escape_radius, capture_margin=1e-3, mass=1.0,
+                         score_factor=3.0):
+    """The terminal points with every ray's accepted step sizes given
+    (``h_lists[i]``: its accepted steps' sizes, in order): exactly the map
+    the discrete adjoint differentiates.  Central differences of this
+    function pin the adjoint without the controller's response."""
+    a_val = float(spin.detach()) if isinstance(spin, torch.Tensor) else float(spin)
+    r_plus = mass + (mass * mass - a_val * a_val) ** 0.5
+    capture_radius = r_plus * (1.0 + capture_margin)
+    r_score = r_plus * score_factor
+    rows = []
+    for i in range(state0.shape[0]):
+        y = state0[i:i + 1]
+        r_prev = float(radius(a_val, y.detach()[:, 1:4]))
+        score_ev = last_ev = None
+        for h in h_lists[i].tolist():
+            new, _ = dp5(mass, spin, y, torch.tensor([h], dtype=torch.float64))
+            r_new = float(radius(a_val, new.detach()[:, 1:4]))
+            if score_ev is None and r_prev > r_score and r_new <= r_score:
+                score_ev = (y, new, h)
+            last_ev = (y, new, h)
+            y, r_prev = new, r_new
+        if last_ev is not None and r_prev >= escape_radius:
+            y0, y1, h = last_ev
+            R = torch.tensor([float(escape_radius)], dtype=torch.float64)
+        elif r_prev <= capture_radius and score_ev is not None:
+            y0, y1, h = score_ev
+            sp = spin if isinstance(spin, torch.Tensor) else torch.tensor(float(spin),
+                                                                          dtype=torch.float64)
+            R = ((mass + torch.sqrt(mass * mass - sp * sp)) * score_factor).reshape(1)
+        else:
+            rows.append(y)
+            continue
+        rows.append(_event_point(mass, spin, a_val, y0, y1,
+                                 torch.tensor([h], dtype=torch.float64), R))
+    return torch.cat(rows, 0)

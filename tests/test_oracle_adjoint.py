@@ -1,6 +1,7 @@
 """Pins the autograd derivative oracle: its forward agrees with the NumPy
-oracle (statuses, accepted steps) and its gradients with central finite
-differences of its own terminal points."""
+oracle (statuses, accepted steps); its gradients equal central differences
+of the same discrete map (accepted step sizes held fixed) and, on escaped
+rays, central differences of the full adaptive trace."""
 
 import numpy as np
 import pytest
@@ -34,32 +35,47 @@ def test_forward_matches_numpy_oracle():
     assert np.all(np.abs(r - 45.0) < 0.5)  # terminal points sit on the escape sphere
 
 
-@pytest.mark.parametrize("which", ["state", "spin"])
-def test_gradients_match_central_differences(which):
+def test_gradients_equal_fixed_step_central_differences():
     s0 = _rays(0.7, n=4, seed=2)
     w = torch.from_numpy(np.random.default_rng(3).normal(size=(4, 8)))
-    w[:, 0] = 0.0  # coordinate time is not part of a direction loss
-
-    def f(state, spin):
-        y, st, _ = ado.trace_terminal(state, spin, **KW)
-        return (y * w).sum(), st
-
+    w[:, 0] = 0.0
     st0 = torch.from_numpy(s0).requires_grad_(True)
     sp0 = torch.tensor(0.7, dtype=torch.float64, requires_grad=True)
-    L, status = f(st0, sp0)
-    gs, ga = torch.autograd.grad(L, [st0, sp0])
+    y, status, _, hl = ado.trace_terminal(st0, sp0, return_steps=True, **KW)
+    assert set(status.tolist()) == {1, 2}
+    gs, ga = torch.autograd.grad((y * w).sum(), [st0, sp0])
+    kw = dict(escape_radius=KW["escape_radius"], capture_margin=KW["capture_margin"])
+    with torch.no_grad():
+        base = ado.terminal_fixed_steps(torch.from_numpy(s0), 0.7, hl, **kw)
+        np.testing.assert_allclose(base.numpy(), y.detach().numpy(), rtol=1e-12, atol=1e-12)
+        f = lambda s, a: float((ado.terminal_fixed_steps(s, a, hl, **kw) * w).sum())  # noqa
+        eps = 1e-6
+        for i, j in [(0, 5), (1, 6), (2, 7), (3, 1), (0, 3), (2, 2)]:
+            d = np.zeros_like(s0)
+            d[i, j] = eps
+            fd = (f(torch.from_numpy(s0 + d), 0.7) - f(torch.from_numpy(s0 - d), 0.7)) / (2 * eps)
+            assert abs(fd - gs[i, j].item()) <= 1e-6 * abs(fd) + 1e-8, (i, j, fd, gs[i, j])
+        sa = lambda a: f(torch.from_numpy(s0), torch.tensor(a, dtype=torch.float64))  # noqa
+        fd = (sa(0.7 + eps) - sa(0.7 - eps)) / (2 * eps)
+        assert abs(fd - ga.item()) <= 1e-6 * abs(fd)
+
+
+def test_escaped_ray_gradients_match_adaptive_central_differences():
+    """On escaped rays the controller's own response is O(rtol)."""
+    s0 = _rays(0.7, n=4, seed=2)
+    w = torch.from_numpy(np.random.default_rng(3).normal(size=(4, 8)))
+    w[:, 0] = 0.0
+    st0 = torch.from_numpy(s0).requires_grad_(True)
+    y, status, _ = ado.trace_terminal(st0, torch.tensor(0.7, dtype=torch.float64), **KW)
+    (gs,) = torch.autograd.grad((y * w).sum(), [st0])
+    esc = np.nonzero(status.numpy() == 2)[0]
     eps = 1e-6
     with torch.no_grad():
-        if which == "spin":
-            lp, _ = f(torch.from_numpy(s0), torch.tensor(0.7 + eps, dtype=torch.float64))
-            lm, _ = f(torch.from_numpy(s0), torch.tensor(0.7 - eps, dtype=torch.float64))
-            fd = (lp - lm).item() / (2 * eps)
-            assert abs(fd - ga.item()) <= 1e-5 * abs(fd) + 1e-9
-        else:
-            for i, j in [(0, 5), (1, 6), (2, 7), (3, 1), (0, 3)]:
+        f = lambda s: float((ado.trace_terminal(s, torch.tensor(0.7, dtype=torch.float64),  # noqa
+                                                **KW)[0] * w).sum())
+        for i in esc:
+            for j in (3, 5, 6):
                 d = np.zeros_like(s0)
                 d[i, j] = eps
-                lp, _ = f(torch.from_numpy(s0 + d), torch.tensor(0.7, dtype=torch.float64))
-                lm, _ = f(torch.from_numpy(s0 - d), torch.tensor(0.7, dtype=torch.float64))
-                fd = (lp - lm).item() / (2 * eps)
-                assert abs(fd - gs[i, j].item()) <= 1e-5 * abs(fd) + 1e-7, (i, j, fd, gs[i, j])
+                fd = (f(torch.from_numpy(s0 + d)) - f(torch.from_numpy(s0 - d))) / (2 * eps)
+                assert abs(fd - gs[i, j].item()) <= 1e-5 * abs(fd) + 1e-8, (i, j, fd, gs[i, j])
