@@ -321,13 +321,88 @@ def run_shadowfit(args) -> dict:
                 status_counts=torch.bincount(status.long(), minlength=7).tolist())
 
 
+def run_orbits(args) -> dict:
+    """65,536 near-circular timelike orbits around a = 0.9 in 16 uneven jobs,
+    integrated to proper time 1e4 M at rtol 1e-10 in one persistent batched
+    launch; E / L_z / H drift; adjoint of the final positions w.r.t. spin and
+    initial momenta."""
+    import torch
+
+    from blackhorizon_amd import _lib, adjoint, batch, geodesics, kerr, workloads
+
+    _lib.require_device()
+    jobs = workloads.orbit_batches(n_total=args.orbits, lambda_end=args.proper_time)
+    st = kerr.KerrSpacetime(1.0, 0.9)
+    tjobs, states = [], []
+    for j in jobs:
+        pos = torch.from_numpy(j.positions).cuda()
+        mom = geodesics.timelike_momentum(st, pos, torch.from_numpy(j.velocities).cuda())
+        s0 = geodesics.build_state(pos, mom)
+        states.append(s0)
+        tjobs.append(batch.TraceJob(st, s0, j.escape_radius, j.max_steps, j.rtol, j.atol,
+                                    j.initial_step, j.max_step, j.min_step, j.capture_margin,
+                                    j.lambda_end))
+    plan = batch.BatchPlan(tjobs, mode=args.mode)
+    cur = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for _ in range(args.warmup):
+        plan.launch()
+    torch.cuda.synchronize()
+    e0.record(cur)
+    for _ in range(args.steps):
+        plan.launch()
+    e1.record(cur)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    print("[bench] orbits forward done", file=sys.stderr, flush=True)
+    res = plan.results(as_numpy=False)
+    s0 = torch.cat(states)
+    s1 = torch.cat([r.states for r in res])
+    status = torch.cat([r.status for r in res])
+    steps = int(sum(int(r.steps.sum().item()) for r in res))
+    d0 = geodesics._diagnostics(st, s0, ("ham", "energy", "lz"))
+    d1 = geodesics._diagnostics(st, s1, ("ham", "energy", "lz"))
+    ok = status == 6
+    dlz = ((d1["lz"] - d0["lz"]).abs() / d0["lz"].abs())[ok]
+    dh = (d1["ham"] - d0["ham"]).abs()[ok]
+    out = dict(orbits=int(s0.shape[0]), jobs=len(jobs), proper_time=args.proper_time,
+               ray_steps=steps, ms_forward=ms, ray_steps_per_s=steps / (ms * 1e-3),
+               status_counts=torch.bincount(status.long(), minlength=7).tolist(),
+               energy_drift_max=float((d1["energy"] - d0["energy"]).abs().max()),
+               lz_rel_drift_max=float(dlz.max()), lz_rel_drift_median=float(dlz.median()),
+               ham_drift_max=float(dh.max()), ham_drift_median=float(dh.median()))
+    # adjoint of the final positions (one recorded launch over all orbits)
+    cfg = adjoint.TraceSettings(escape_radius=1000.0, max_steps=10_000_000, rtol=1e-10,
+                                atol=1e-12, initial_step=0.05, min_step=1e-14, K=32,
+                                lambda_end=args.proper_time, mode=args.mode)
+    sg = s0.clone().requires_grad_(True)
+    a = torch.tensor(0.9, dtype=torch.float64, device="cuda", requires_grad=True)
+    w = torch.randn((sg.shape[0], 3), dtype=torch.float64, device="cuda",
+                    generator=torch.Generator(device="cuda").manual_seed(3))
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    e[0].record(cur)
+    Y, st_ad = adjoint.trace_terminal(sg, a, cfg)
+    e[1].record(cur)
+    (Y[:, 1:4] * w).sum().backward()
+    e[2].record(cur)
+    torch.cuda.synchronize()
+    out.update(ms_recording_forward=e[0].elapsed_time(e[1]), ms_adjoint=e[1].elapsed_time(e[2]),
+               adjoint_over_forward=e[1].elapsed_time(e[2]) / ms,
+               grad_spin=float(a.grad),
+               grad_momentum_norm_median=float(sg.grad[:, 5:8].norm(dim=1).median()),
+               grad_finite=bool(torch.isfinite(sg.grad).all()))
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="hd",
-                    choices=["hd", "hd09", "small", "uneven", "shadowfit"])
+                    choices=["hd", "hd09", "small", "uneven", "shadowfit", "orbits"])
+    ap.add_argument("--orbits", type=int, default=65536, help="orbits: number of orbits")
+    ap.add_argument("--proper-time", type=float, default=1.0e4, help="orbits: proper time")
     ap.add_argument("--fit-size", type=int, default=512, help="shadowfit: image side")
     ap.add_argument("--serial-groups", action="store_true",
                     help="uneven: run the fast and precise batches one after the other")
@@ -348,6 +423,11 @@ def main(argv=None) -> int:
                                    "measured; run with --gpus 1"}))
         return 2
 
+    if args.workload == "orbits":
+        res = run_orbits(args)
+        print(json.dumps({"metric": METRIC, "workload": "timelike orbits", "mode": args.mode,
+                          **res}), flush=True)
+        return 0
     if args.workload == "shadowfit":
         res = run_shadowfit(args)
         print(json.dumps({"metric": METRIC, "workload": "shadow fit", "mode": args.mode,

@@ -42,6 +42,7 @@ class TraceParams(ctypes.Structure):
         ("max_steps", ctypes.c_int64),
         ("mode", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+        ("lambda_end", ctypes.c_double),
     ]
 
 

@@ -32,9 +32,11 @@ import torch
 
 from . import _lib
 from .kerr import KerrSpacetime
-from .tracer import make_params
+from .tracer import make_params, trace_rays
 
 TINY = 1e-30
+CKPT_VALUES = 10              # state (8), controller step size, affine parameter
+CKPT_BUDGET_BYTES = 16 << 30  # checkpoint memory reserved from max_steps alone
 
 
 @dataclass
@@ -56,6 +58,7 @@ class TraceSettings:
     K: int = 32
     cap: int | None = None
     score_factor: float = 3.0
+    lambda_end: float = math.inf
 
 
 class Recorded:
@@ -70,9 +73,18 @@ class Recorded:
         st = KerrSpacetime(cfg.mass, spin)
         self.params = make_params(st, cfg.escape_radius, cfg.max_steps, cfg.rtol, cfg.atol,
                                   cfg.initial_step, cfg.max_step, cfg.min_step,
-                                  cfg.capture_margin, cfg.mode)
+                                  cfg.capture_margin, cfg.mode, cfg.lambda_end)
         self.score_radius = cfg.score_factor * st.outer_horizon_radius
         cap = cfg.cap if cfg.cap is not None else cfg.max_steps // cfg.K + 2
+        if cfg.cap is None and cap * CKPT_VALUES * 8 * max(n, 1) > CKPT_BUDGET_BYTES:
+            # max_steps would reserve more than the budget: size the
+            # checkpoints from the step counts of a plain forward instead
+            pre = trace_rays(st, state0, cfg.escape_radius, max_steps=cfg.max_steps,
+                             rtol=cfg.rtol, atol=cfg.atol, initial_step=cfg.initial_step,
+                             max_step=cfg.max_step, min_step=cfg.min_step,
+                             capture_margin=cfg.capture_margin, mode=cfg.mode,
+                             lambda_end=cfg.lambda_end)
+            cap = int(pre.steps.max().item()) // cfg.K + 2 if n else 1
         self.cap = max(int(cap), 1)
         ld = max(n, 1)
         f64 = dict(dtype=torch.float64, device=dev)
@@ -90,7 +102,7 @@ class Recorded:
         self.steps = torch.empty(n, dtype=torch.int64, device=dev)
         self.attempts = torch.empty(n, dtype=torch.int64, device=dev)
         self.iters = torch.zeros(1, dtype=torch.int64, device=dev)
-        self.ckpt = torch.empty((self.cap, 9, ld), **f64)
+        self.ckpt = torch.empty((self.cap, CKPT_VALUES, ld), **f64)
         self.n_ckpt = torch.empty(n, dtype=torch.int32, device=dev)
         self.term = torch.empty((8, ld), **f64)
         self.prev = torch.empty((9, ld), **f64)

@@ -16,6 +16,7 @@ so it can be timed or captured; ``trace_batch`` is the one-call form.
 from __future__ import annotations
 
 import ctypes
+import math
 from dataclasses import dataclass
 
 import numpy
@@ -31,7 +32,8 @@ MAX_JOBS_PER_LAUNCH = 256
 
 @dataclass
 class TraceJob:
-    """One trace_rays call's arguments (tracer.py:50-61)."""
+    """One trace_rays call's arguments (tracer.py:50-61), plus an optional
+    affine-parameter end (see tracer.integrate_rays)."""
 
     spacetime: KerrSpacetime
     state0: Array
@@ -43,6 +45,7 @@ class TraceJob:
     max_step: float = 2.0
     min_step: float = 1e-10
     capture_margin: float = 1e-3
+    lambda_end: float = math.inf
 
 
 class _Group:
@@ -64,7 +67,8 @@ class _Group:
         for k, i in enumerate(idx):
             j = jobs[i]
             p = make_params(j.spacetime, j.escape_radius, j.max_steps, j.rtol, j.atol,
-                            j.initial_step, j.max_step, j.min_step, j.capture_margin, mode)
+                            j.initial_step, j.max_step, j.min_step, j.capture_margin, mode,
+                            j.lambda_end)
             params.append(p)
             lo = int(self.offsets_host[k])
             if counts[k]:

@@ -23,6 +23,7 @@ struct JobConst {
   double initial_step, max_step, min_step;
   int64_t max_steps;
   int64_t max_attempts;  // 4 * max_steps  (tracer.py:101)
+  double lambda_end;     // +inf when the caller sets none
 };
 
 __host__ __device__ inline JobConst make_job_const(const bh_trace_params &p) {
@@ -42,6 +43,7 @@ __host__ __device__ inline JobConst make_job_const(const bh_trace_params &p) {
   c.min_step = p.min_step;
   c.max_steps = p.max_steps;
   c.max_attempts = 4 * p.max_steps;
+  c.lambda_end = p.lambda_end > 0.0 ? p.lambda_end : __builtin_inf();  // <= 0 or NaN: none
   return c;
 }
 
@@ -58,9 +60,12 @@ struct TraceIO {
   int64_t n;
 };
 
+// Values per checkpoint: state (8), controller step size, affine parameter.
+constexpr int CKPT_VALUES = 10;
+
 // Forward-with-checkpoints outputs (SoA, plane stride n).
 struct RecordIO {
-  double *ckpt;      // [cap][9][n]: state (8) and controller step size every K accepted steps
+  double *ckpt;      // [cap][CKPT_VALUES][n] every K accepted steps
   int32_t *n_ckpt;   // checkpoints written per ray
   double *term;      // [8][n]: terminal point
   double *prev;      // [9][n]: start state and step size of the event step

@@ -9,6 +9,7 @@ moves buffers and reads back ``iterations``.
 
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass
 from enum import IntEnum
 
@@ -71,13 +72,14 @@ def resolve_mode(mode: str, rtol: float) -> str:
 
 def make_params(spacetime: KerrSpacetime, escape_radius: float, max_steps: int, rtol: float,
                 atol: float, initial_step: float, max_step: float, min_step: float,
-                capture_margin: float, mode: str = "auto") -> _lib.TraceParams:
+                capture_margin: float, mode: str = "auto",
+                lambda_end: float = math.inf) -> _lib.TraceParams:
     # tracer.py:84 computes the capture radius in host double precision
     capture_radius = spacetime.outer_horizon_radius * (1.0 + capture_margin)
     return _lib.TraceParams(spacetime.mass, spacetime.spin, float(escape_radius),
                             capture_radius, float(rtol), float(atol), float(initial_step),
                             float(max_step), float(min_step), int(max_steps),
-                            _mode(resolve_mode(mode, rtol)), 0)
+                            _mode(resolve_mode(mode, rtol)), 0, float(lambda_end))
 
 
 def _order_tensor(ray_order, n: int, dev):
@@ -107,6 +109,7 @@ def trace_rays(
     mode: str = "auto",
     tune: Tune | None = None,
     ray_order: Array | None = None,
+    lambda_end: float = math.inf,
 ) -> TraceResult:
     """Trace a batch of geodesics to termination (tracer.py:50-157).
 
@@ -120,9 +123,11 @@ def trace_rays(
     would move final states by more than 1e-7.  ``ray_order`` (a
     permutation) sets which rays share a wavefront -- e.g.
     ``PinholeCamera.tile_order()`` -- without changing any result.
+    ``lambda_end`` (affine parameter) ends rays that reach it with status
+    TERMINATED (see integrate_rays).
     """
     params = make_params(spacetime, escape_radius, max_steps, rtol, atol, initial_step,
-                         max_step, min_step, capture_margin, mode)
+                         max_step, min_step, capture_margin, mode, lambda_end)
     s0 = _dev.as_device_f64(state0, (8,), "state0")
     n = s0.shape[0]
     dev = s0.device
@@ -156,3 +161,39 @@ def trace_rays(
                        steps=_dev.like_input(steps, state0),
                        iterations=iterations,
                        attempts=_dev.like_input(attempts, state0))
+
+
+def integrate_rays(
+    spacetime: KerrSpacetime,
+    state0: Array,
+    lambda_total: float,
+    rtol: float = 1e-10,
+    atol: float = 1e-12,
+    initial_step: float = 0.05,
+    max_step: float = 2.0,
+    min_step: float = 1e-14,
+    max_steps: int = 10_000_000,
+    escape_radius: float = math.inf,
+    capture_margin: float = 1e-3,
+    *,
+    mode: str = "auto",
+    tune: Tune | None = None,
+) -> TraceResult:
+    """Integrate every ray over the affine interval [0, lambda_total] with
+    its own adaptive step: per ray, exactly integrate_dense
+    (integrators.py:127-174) run on that ray alone -- steps clipped to
+    min(h, max_step, lambda_total - lambda), the same error ratio and step
+    factor, defaults rtol 1e-10, atol 1e-12, initial_step 0.05 -- except
+    that each ray keeps its own step instead of the batch maximum.  Rays
+    that reach lambda_total end TERMINATED; a ray whose step falls below
+    ``min_step`` (integrate_dense raises at 1e-14) ends FAILED; rays that
+    fall through the horizon end CAPTURED (and, with a finite
+    ``escape_radius``, rays that leave end ESCAPED).  For timelike rays
+    normalised to H = -1/2 the affine parameter is proper time."""
+    if not lambda_total > 0.0:
+        raise ValueError("lambda_total must be positive")
+    return trace_rays(spacetime, state0, escape_radius, max_steps=max_steps, rtol=rtol,
+                      atol=atol, initial_step=min(initial_step, max_step), max_step=max_step,
+                      min_step=min_step, capture_margin=capture_margin, mode=mode, tune=tune,
+                      lambda_end=float(lambda_total))
+

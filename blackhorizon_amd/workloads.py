@@ -115,3 +115,77 @@ def uneven_mix(n_jobs: int = 64, seed: int = UNEVEN_MIX_SEED,
                               escape_radius=2.0 * r_obs, rtol=rtol, atol=1e-2 * rtol,
                               capture_margin=_margin_for_absolute(spin)))
     return jobs
+
+
+@dataclass(frozen=True)
+class OrbitJob:
+    """Timelike orbits of one batch job: positions and coordinate velocities
+    (dx^i/dt) of unit-mass particles, integrated over proper time."""
+
+    spin: float
+    positions: np.ndarray
+    velocities: np.ndarray
+    lambda_end: float
+    rtol: float
+    atol: float = 1e-12
+    initial_step: float = 0.05
+    max_step: float = 2.0
+    min_step: float = 1e-14
+    max_steps: int = 10_000_000
+    escape_radius: float = 1000.0
+    capture_margin: float = 1e-3
+    mass: float = 1.0
+
+    @property
+    def n_rays(self) -> int:
+        return self.positions.shape[0]
+
+
+ORBIT_SEED = 5
+
+
+def near_circular_orbits(n: int, spin: float = 0.9, seed: int = ORBIT_SEED, r_min: float = 6.0,
+                         r_max: float = 50.0, spread: float = 0.05):
+    """Seeded near-circular orbits: radius U[6, 50] M, prograde or retrograde
+    with equal probability at the Boyer-Lindquist circular angular velocity
+    Omega = +-1 / (r^1.5 +- a) (the reference's test_orbits.py:51-54 and
+    118-120), every velocity component scaled by 1 + spread N(0, 1), the
+    orbit plane tilted about the x axis by an inclination U[0, 90] deg.
+    Positions follow x + i y = (r + i a) e^{i phi} in the plane, velocity
+    Omega d/dphi of it.  Returns (positions (n, 3), dx^i/dt (n, 3))."""
+    rng = np.random.default_rng(seed)
+    r = rng.uniform(r_min, r_max, n)
+    phi = rng.uniform(0.0, 2.0 * math.pi, n)
+    pro = rng.random(n) < 0.5
+    a = spin
+    omega = np.where(pro, 1.0 / (r ** 1.5 + a), -1.0 / (r ** 1.5 - a))
+    x = r * np.cos(phi) - a * np.sin(phi)
+    y = r * np.sin(phi) + a * np.cos(phi)
+    vx = -omega * y
+    vy = omega * x
+    v = np.stack([vx, vy, np.zeros(n)], -1) * (1.0 + spread * rng.normal(size=(n, 3)))
+    p = np.stack([x, y, np.zeros(n)], -1)
+    inc = np.radians(rng.uniform(0.0, 90.0, n))
+    c, s = np.cos(inc), np.sin(inc)
+
+    def tilt(u):
+        return np.stack([u[:, 0], u[:, 1] * c - u[:, 2] * s, u[:, 1] * s + u[:, 2] * c], -1)
+
+    return tilt(p), tilt(v)
+
+
+def orbit_batches(n_total: int = 65536, n_jobs: int = 16, spin: float = 0.9,
+                  lambda_end: float = 1.0e4, rtol: float = 1e-10,
+                  seed: int = ORBIT_SEED) -> list[OrbitJob]:
+    """65,536 near-circular timelike orbits around a = 0.9 split into 16
+    jobs of seeded uneven sizes (log-uniform weights), each integrated to
+    proper time 1e4 M at rtol 1e-10."""
+    pos, vel = near_circular_orbits(n_total, spin, seed)
+    rng = np.random.default_rng(seed + 1)
+    w = np.exp(rng.uniform(0.0, math.log(30.0), n_jobs))
+    sizes = np.maximum(1, np.floor(w / w.sum() * n_total)).astype(int)
+    sizes[-1] = n_total - sizes[:-1].sum()
+    edges = np.concatenate([[0], np.cumsum(sizes)])
+    return [OrbitJob(spin=spin, positions=pos[edges[j]:edges[j + 1]],
+                     velocities=vel[edges[j]:edges[j + 1]], lambda_end=lambda_end, rtol=rtol)
+            for j in range(n_jobs)]

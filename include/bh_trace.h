@@ -69,6 +69,12 @@ typedef struct bh_trace_params {
   int64_t max_steps;   /* accepted-step cap; attempts are capped at 4 * max_steps */
   int32_t mode;        /* BH_MODE_FAST or BH_MODE_PRECISE */
   int32_t reserved;
+  /* Affine-parameter end; <= 0, NaN or +inf: none (so a zero-initialised
+   * struct traces like trace_rays).  Each attempt uses
+   * min(h, lambda_end - lambda) as integrate_dense does
+   * (integrators.py:161-162); a ray reaching lambda_end ends with status
+   * BH_TERMINATED.  For timelike rays with H = -1/2 lambda is proper time. */
+  double lambda_end;
 } bh_trace_params;
 
 /* Launch shape of the plain tracer.  Zero fields pick defaults. */
@@ -140,7 +146,8 @@ int32_t bh_adjoint_scratch_doubles(int32_t K);
 
 /* Forward trace exactly as bh_trace_plain_f64 (same statuses, steps and
  * final states; one ray per lane), recording for the adjoint:
- *  ckpt  [cap][9][n]: state and controller step size every K accepted steps;
+ *  ckpt  [cap][10][n]: state, controller step size and affine parameter
+ *  every K accepted steps;
  *  n_ckpt [n]; term [8][n]: terminal point -- escaped rays: where the cubic
  *  Hermite dense output of the last step crosses the escape sphere;
  *  captured rays: where the step that first enters the sphere

@@ -142,7 +142,7 @@ def _hermite(th, h, y0, f0, y1, f1, deriv=False):
 
 def trace_terminal(state0, spin, escape_radius, max_steps=20000, rtol=1e-9, atol=1e-12,
                    initial_step=0.1, max_step=2.0, min_step=1e-10, capture_margin=1e-3,
-                   mass=1.0, score_factor=3.0, return_steps=False):
+                   mass=1.0, score_factor=3.0, return_steps=False, lambda_end=float("inf")):
     """Terminal points (n, 8) and statuses of a trace, differentiable in
     ``state0`` and ``spin`` through the accepted steps (fixed step sizes).
 
@@ -167,21 +167,24 @@ def trace_terminal(state0, spin, escape_radius, max_steps=20000, rtol=1e-9, atol
     crossed = torch.zeros(n, dtype=torch.bool)
     ys0, ys1, hs = state0, state0, torch.zeros(n, dtype=torch.float64)
     r_cur = radius(a_val, state0.detach()[:, 1:4])
+    lam = torch.zeros(n, dtype=torch.float64)
     h_hist = []
     for _ in range(4 * max_steps):
         active = status == 0
         if not bool(active.any()):
             break
         y_in = torch.where(active[:, None], y, dummy)
-        new, err = dp5(mass, spin, y_in, h)
+        ht = torch.minimum(h, lambda_end - lam)  # integrate_dense's clip to the end
+        new, err = dp5(mass, spin, y_in, ht)
         ratio = error_ratio(y_in.detach(), new.detach(), err.detach(), rtol, atol)
         accept = (ratio <= 1.0) & active
         y_prev = torch.where(accept[:, None], y, y_prev)
-        h_prev = torch.where(accept, h, h_prev)
-        h_hist.append(torch.where(accept, h, torch.full_like(h, float("nan"))))
+        h_prev = torch.where(accept, ht, h_prev)
+        h_hist.append(torch.where(accept, ht, torch.full_like(ht, float("nan"))))
         y = torch.where(accept[:, None], new, y)
         steps = steps + accept.to(torch.int64)
-        h = torch.where(active, torch.clamp(h * step_factor(ratio), 0.0, max_step), h)
+        lam = torch.where(accept, lam + ht, lam)
+        h = torch.where(active, torch.clamp(ht * step_factor(ratio), 0.0, max_step), h)
         yd = y.detach()
         rad = radius(a_val, yd[:, 1:4])
         first = accept & ~crossed & (r_cur > r_score) & (rad <= r_score)
@@ -198,6 +201,7 @@ def trace_terminal(state0, spin, escape_radius, max_steps=20000, rtol=1e-9, atol
         still = active & (s == 0)
         s = torch.where(still & (h < min_step), torch.full_like(s, 4), s)
         s = torch.where(still & (steps >= max_steps), torch.full_like(s, 3), s)
+        s = torch.where((s == 0) & (lam >= lambda_end), torch.full_like(s, 6), s)
         status = s
     status = torch.where(status == 0, torch.full_like(status, 3), status)
     esc = (status == 2) & (steps > 0)

@@ -381,8 +381,11 @@ def step_factor_ulp_noise(seed: int):
 
 
 def integrate_dense(rhs, state0, lambda_total, rtol=1e-10, atol=1e-12,
-                    initial_step=0.05, max_step=2.0):
-    """Batch-shared adaptive DP5 over an affine interval; integrators.py:127-174."""
+                    initial_step=0.05, max_step=2.0, factor_fn=None):
+    """Batch-shared adaptive DP5 over an affine interval; integrators.py:127-174.
+
+    ``factor_fn(ratio) -> factor`` replaces the step-factor expression (the
+    reference evaluates it with Python floats, i.e. libm pow)."""
     n = state0.shape[0]
     state = state0.copy()
     states = [state.copy()]
@@ -397,7 +400,10 @@ def integrate_dense(rhs, state0, lambda_total, rtol=1e-10, atol=1e-12,
             state = new_state
             states.append(state.copy())
             elapsed += h_scalar
-        factor = float(min(max(0.9 * max(ratio, 1e-16) ** (-0.2), 0.2), 5.0))
+        if factor_fn is None:
+            factor = float(min(max(0.9 * max(ratio, 1e-16) ** (-0.2), 0.2), 5.0))
+        else:
+            factor = float(factor_fn(np.array([ratio]))[0])
         h_scalar *= factor
         if h_scalar < 1e-14:
             raise RuntimeError("integrate_dense: step size underflow")

@@ -2,11 +2,12 @@
 
 Run in a container that holds the reference checkout:
 
-    python tests/golden/make_golden.py /root/reference
+    python tests/golden/make_golden.py /root/reference [part ...]
 
-Only the reference's ``trace_rays`` (tracer.py:50-157) is executed: it is the
-one function on the path that no closed form or invariant pins to the
-per-ray level (final states, status, accepted steps, iterations). Its input
+Only the reference's ``trace_rays`` (tracer.py:50-157) and
+``integrate_dense`` (integrators.py:127-174) are executed: they are the
+functions on the path that no closed form or invariant pins to the per-ray
+level (final states, status, accepted steps, iterations). Its input
 states are built by the oracle (camera + null momenta), so the fixtures hold
 plain data: ``state0`` rows and the reference outputs for them. Rays are
 independent in ``trace_rays``, so a subsample of a frame has the same per-ray
@@ -185,18 +186,72 @@ def shadow_png_classes(ref_root):
     print("shadow_gpu_classes", np.bincount(cls.ravel(), minlength=4))
 
 
+def dense_orbits(integrate_dense, geodesic_rhs, ref_kerr):
+    """integrate_dense (integrators.py:127-174) run on one ray at a time,
+    where its batch-shared step is that ray's own: the per-ray reference for
+    integrating to a fixed affine parameter.  Rays: the photon flyby and the
+    circular r = 6 M orbit of the reference's test_orbits.py (22-46,
+    108-136) and a few seeded near-circular orbits around a = 0.9."""
+    st = orc.Kerr(spin=0.9)
+    cases = []
+    pos = np.array([[30.0, 4.0, 1.5]])
+    mom = orc.null_momentum_from_velocity(st, pos, np.array([[-1.0, 0.0, 0.0]]), "future")
+    cases.append((orc.build_state(pos, mom), 70.0, 1e-11))
+    om = 1.0 / (6.0 ** 1.5 + 0.9)
+    pos = np.array([[6.0, 0.9, 0.0]])
+    mom = orc.timelike_momentum(st, pos, np.array([[-om * 0.9, om * 6.0, 0.0]]))
+    cases.append((orc.build_state(pos, mom), 170.0, 1e-11))
+    p, v = workloads.near_circular_orbits(12, seed=11)
+    for i in range(12):
+        mom = orc.timelike_momentum(st, p[i:i + 1], v[i:i + 1])
+        cases.append((orc.build_state(p[i:i + 1], mom), 400.0, 1e-10))
+    out = {}
+    kst = ref_kerr(spin=0.9)
+    k = 0
+    for s0, lam, rtol in cases:
+        try:
+            states = integrate_dense(lambda y: geodesic_rhs(kst, y), s0, lambda_total=lam,
+                                     rtol=rtol)
+        except RuntimeError as e:
+            # the last step can leave a remainder of a few ulp of lambda, whose
+            # follow-up step underflows; whether that happens is decided by
+            # rounding, so such rays are not fixtures
+            print("skipped:", e)
+            continue
+        out[f"c{k}_state0"] = s0
+        out[f"c{k}_final"] = np.asarray(states[-1])
+        out[f"c{k}_steps"] = np.int64(len(states) - 1)
+        out[f"c{k}_lambda"] = np.float64(lam)
+        out[f"c{k}_rtol"] = np.float64(rtol)
+        print("dense orbit", k, len(states) - 1)
+        k += 1
+    np.savez_compressed(os.path.join(HERE, "dense_orbits.npz"), **out)
+
+
 def main():
     ref_root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    parts = sys.argv[2:] or ["png", "edge", "fans", "small", "uneven", "hd", "dense"]
     sys.path.insert(0, ref_root)
+    from blackhorizon.geodesics import geodesic_rhs
+    from blackhorizon.integrators import integrate_dense
     from blackhorizon.kerr import KerrSpacetime
     from blackhorizon.tracer import trace_rays
 
-    shadow_png_classes(ref_root)
-    edge_cases(trace_rays, KerrSpacetime)
-    fans(trace_rays, KerrSpacetime)
-    frame_subsample(trace_rays, KerrSpacetime, workloads.small_frame(), 8, "small_frame_stride8.npz")
-    uneven_subsample(trace_rays, KerrSpacetime)
-    hd_subsample(trace_rays, KerrSpacetime)
+    if "png" in parts:
+        shadow_png_classes(ref_root)
+    if "edge" in parts:
+        edge_cases(trace_rays, KerrSpacetime)
+    if "fans" in parts:
+        fans(trace_rays, KerrSpacetime)
+    if "small" in parts:
+        frame_subsample(trace_rays, KerrSpacetime, workloads.small_frame(), 8,
+                        "small_frame_stride8.npz")
+    if "uneven" in parts:
+        uneven_subsample(trace_rays, KerrSpacetime)
+    if "hd" in parts:
+        hd_subsample(trace_rays, KerrSpacetime)
+    if "dense" in parts:
+        dense_orbits(integrate_dense, geodesic_rhs, KerrSpacetime)
 
 
 if __name__ == "__main__":

@@ -20,7 +20,8 @@ def test_library_exports_header_symbols():
 
 def _params(**over):
     p = dict(mass=1.0, spin=0.9, escape_radius=45.0, capture_radius=1.5, rtol=1e-8, atol=1e-10,
-             initial_step=0.1, max_step=2.0, min_step=1e-10, max_steps=20000, mode=0, reserved=0)
+             initial_step=0.1, max_step=2.0, min_step=1e-10, max_steps=20000, mode=0, reserved=0,
+             lambda_end=float("inf"))
     p.update(over)
     return _lib.TraceParams(**p)
 

@@ -1,0 +1,135 @@
+"""GPU: integration to a fixed affine parameter (timelike orbits, proper
+time) against the reference's integrate_dense run per ray (golden vectors),
+batched jobs against plain launches, and the adjoint with an affine end
+against the autograd oracle."""
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import golden
+from oracle import ad_oracle as ado
+from oracle import bh_oracle as orc
+from parity import assert_trace_parity, block_diff
+from blackhorizon_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _golden_cases():
+    g = golden("dense_orbits.npz")
+    k = 0
+    out = []
+    while f"c{k}_state0" in g.files:
+        out.append((g[f"c{k}_state0"], float(g[f"c{k}_lambda"]), float(g[f"c{k}_rtol"]),
+                    g[f"c{k}_final"], int(g[f"c{k}_steps"])))
+        k += 1
+    return out
+
+
+def _oracle_spread(s0, lam, rtol, final):
+    """How far integrate_dense's own final state moves under ulp-level step
+    factor changes (NumPy SIMD pow, random-ulp pow) -- its conditioning."""
+    st = orc.Kerr(spin=0.9)
+    worst = 0.0
+    for fn in (orc.step_factor, orc.step_factor_ulp_noise(1), orc.step_factor_ulp_noise(2)):
+        alt = orc.integrate_dense(lambda y: orc.geodesic_rhs(st, y), s0, lam, rtol=rtol,
+                                  factor_fn=fn)[-1]
+        worst = max(worst, float(block_diff(alt, final)[0]))
+    return worst
+
+
+@pytest.mark.parametrize("mode", ["auto", "precise"])
+def test_integrate_rays_matches_reference_integrate_dense(hip, mode):
+    from blackhorizon_amd.tracer import integrate_rays
+
+    st = hip.KerrSpacetime(spin=0.9)
+    cases = _golden_cases()
+    assert len(cases) >= 8
+    for s0, lam, rtol, final, steps in cases:
+        res = integrate_rays(st, s0, lam, rtol=rtol, mode=mode)
+        assert int(res.status[0]) == 6, res.status  # TERMINATED at lambda_total
+        spread = _oracle_spread(s0, lam, rtol, final)
+        assert_trace_parity(res.states, res.status, res.steps, final, np.array([6]),
+                            np.array([steps]), label=f"lambda {lam}", spread=np.array([spread]))
+
+
+def test_integrate_rays_conservation(hip):
+    """test_orbits.py tolerances on the final states: E exact, L_z 1e-8
+    (photon flyby, rtol 1e-11) and H within 1e-9; the r = 6 M circular orbit
+    stays circular and in the equatorial plane."""
+    from blackhorizon_amd.tracer import integrate_rays
+
+    st = hip.KerrSpacetime(spin=0.9)
+    ost = orc.Kerr(spin=0.9)
+    cases = _golden_cases()
+    s0, lam, rtol, _, _ = cases[0]
+    res = integrate_rays(st, s0, lam, rtol=rtol)
+    e0, lz0 = orc.conserved_quantities(s0)
+    e1, lz1 = orc.conserved_quantities(res.states)
+    np.testing.assert_array_equal(e1, e0)
+    np.testing.assert_allclose(lz1, lz0, rtol=1e-8)
+    np.testing.assert_allclose(orc.hamiltonian(ost, res.states), 0.0, atol=1e-9)
+    s0, lam, rtol, _, _ = cases[1]
+    res = integrate_rays(st, s0, lam, rtol=rtol)
+    y = res.states
+    np.testing.assert_allclose(ost.kerr_schild_radius(y[:, 1], y[:, 2], y[:, 3]), 6.0, rtol=1e-6)
+    assert abs(y[0, 3]) < 1e-8
+    np.testing.assert_allclose(orc.hamiltonian(ost, y), -0.5, rtol=1e-9)
+
+
+def _orbit_states(job):
+    st = orc.Kerr(spin=job.spin)
+    return orc.build_state(job.positions,
+                           orc.timelike_momentum(st, job.positions, job.velocities))
+
+
+def test_batched_orbit_jobs_equal_plain(hip):
+    from blackhorizon_amd import batch
+    from blackhorizon_amd.tracer import integrate_rays
+
+    jobs = workloads.orbit_batches(n_total=600, n_jobs=5, lambda_end=150.0, rtol=1e-10)
+    tj = [batch.TraceJob(hip.KerrSpacetime(spin=j.spin), _orbit_states(j), j.escape_radius,
+                         max_steps=j.max_steps, rtol=j.rtol, atol=j.atol,
+                         initial_step=j.initial_step, max_step=j.max_step, min_step=j.min_step,
+                         capture_margin=j.capture_margin, lambda_end=j.lambda_end)
+          for j in jobs]
+    res = batch.trace_batch(tj)
+    for j, t, r in zip(jobs, tj, res):
+        alone = integrate_rays(t.spacetime, t.state0, j.lambda_end, rtol=j.rtol, atol=j.atol,
+                               initial_step=j.initial_step, max_step=j.max_step,
+                               min_step=j.min_step, max_steps=j.max_steps,
+                               escape_radius=j.escape_radius, capture_margin=j.capture_margin)
+        np.testing.assert_array_equal(r.status, alone.status)
+        np.testing.assert_array_equal(r.steps, alone.steps)
+        np.testing.assert_array_equal(r.states, alone.states)
+        assert np.all(r.status == 6)
+
+
+def test_adjoint_with_affine_end_matches_autograd_oracle(hip):
+    """Gradients of final positions after proper time 60 M with respect to
+    the initial momenta and the spin (timelike orbits, no terminal event)."""
+    from blackhorizon_amd import adjoint
+
+    p, v = workloads.near_circular_orbits(8, seed=21)
+    s0 = _orbit_states(workloads.OrbitJob(spin=0.9, positions=p, velocities=v,
+                                          lambda_end=60.0, rtol=1e-10))
+    kw = dict(escape_radius=1000.0, rtol=1e-10, atol=1e-12, initial_step=0.05, min_step=1e-14,
+              lambda_end=60.0)
+    w = np.random.default_rng(6).normal(size=(8, 8))
+    w[:, [0, 4]] = 0.0
+    so = torch.from_numpy(s0).requires_grad_(True)
+    ao = torch.tensor(0.9, dtype=torch.float64, requires_grad=True)
+    Yo, sto, _ = ado.trace_terminal(so, ao, max_steps=10_000_000, **kw)
+    assert np.all(sto.numpy() == 6)
+    go_s, go_a = torch.autograd.grad((Yo * torch.from_numpy(w)).sum(), [so, ao])
+    cfg = adjoint.TraceSettings(max_steps=10_000_000, K=16, **kw)
+    sg = torch.from_numpy(s0).cuda().requires_grad_(True)
+    ag = torch.tensor(0.9, dtype=torch.float64, device="cuda", requires_grad=True)
+    Yg, stg = adjoint.trace_terminal(sg, ag, cfg)
+    assert np.all(stg.cpu().numpy() == 6)
+    (Yg * torch.from_numpy(w).cuda()).sum().backward()
+    ref = go_s.numpy()
+    scale = np.maximum(np.abs(ref).max(axis=1, keepdims=True), 1e-12)
+    assert np.max(np.abs(sg.grad.cpu().numpy() - ref) / scale) < 1e-6
+    assert abs(float(ag.grad) - float(go_a)) <= 1e-6 * abs(float(go_a))
