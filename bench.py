@@ -263,7 +263,9 @@ def run_shadowfit(args) -> dict:
     """512 x 512 differentiable shadow fit at a = 0.7, inclination 45 deg,
     r_obs = 40 M, FOV 30 deg, escape sphere 45 M, rtol 1e-8, checkpoints
     every 32 accepted steps; target traced at a = 0.75, inclination 50 deg.
-    Times the plain forward trace, the recording forward and the adjoint."""
+    Times the plain forward trace, the recording forward, the adjoint op
+    (Recorded.backward for the loss's terminal adjoint) and the whole fit
+    step (camera rays, recording trace, loss and autograd gradients)."""
     import math
 
     import torch
@@ -291,31 +293,43 @@ def run_shadowfit(args) -> dict:
     def ev():
         return torch.cuda.Event(enable_timing=True)
 
-    times = {"plain": [], "record": [], "adjoint": []}
+    # the terminal adjoint of the fit loss, for timing the adjoint op alone
+    Y, status = adjoint.trace_terminal(y0, p[0], cfg)
+    loss = adjoint.shadow_loss(Y, status, dirs_t, st_t)
+    gY = torch.autograd.grad(loss, Y, retain_graph=True)[0].detach()
+    y0d = y0.detach().contiguous()
+    times = {"plain": [], "record": [], "adjoint": [], "fit_step": []}
     for k in range(args.warmup + args.steps):
-        e = [ev() for _ in range(4)]
+        e = [ev() for _ in range(6)]
         torch.cuda.synchronize()
         e[0].record(cur)
-        res = tracer.trace_rays(st, y0.detach(), 45.0, rtol=1e-8, atol=1e-10,
+        res = tracer.trace_rays(st, y0d, 45.0, rtol=1e-8, atol=1e-10,
                                 capture_margin=cfg.capture_margin, mode=args.mode)
         e[1].record(cur)
+        rec = adjoint.Recorded(y0d, 0.7, cfg)
+        e[2].record(cur)
+        rec.backward(gY)
+        e[3].record(cur)
+        # the whole differentiable fit step: camera rays, trace, loss, gradients
+        y0 = adjoint.camera_state(p[0], p[1], p[2], 30.0, size, size)
         Y, status = adjoint.trace_terminal(y0, p[0], cfg)
         loss = adjoint.shadow_loss(Y, status, dirs_t, st_t)
-        e[2].record(cur)
-        g = torch.autograd.grad(loss, p, retain_graph=True)
-        e[3].record(cur)
+        g = torch.autograd.grad(loss, p)
+        e[4].record(cur)
         torch.cuda.synchronize()
         if k >= args.warmup:
             times["plain"].append(e[0].elapsed_time(e[1]))
             times["record"].append(e[1].elapsed_time(e[2]))
             times["adjoint"].append(e[2].elapsed_time(e[3]))
+            times["fit_step"].append(e[3].elapsed_time(e[4]))
+        del rec
         print(f"[bench] shadowfit pass {k} done", file=sys.stderr, flush=True)
     med = {k: float(np.median(v)) for k, v in times.items()}
     steps = int(res.steps.sum().item())
     return dict(rays=size * size, ray_steps=steps, loss=float(loss),
                 grad_spin=float(g[0]), grad_inclination=float(g[1]), grad_distance=float(g[2]),
                 ms_plain_forward=med["plain"], ms_recording_forward=med["record"],
-                ms_adjoint=med["adjoint"],
+                ms_adjoint=med["adjoint"], ms_fit_step=med["fit_step"],
                 adjoint_over_forward=med["adjoint"] / med["plain"],
                 forward_plus_adjoint_over_forward=(med["record"] + med["adjoint"]) / med["plain"],
                 status_counts=torch.bincount(status.long(), minlength=7).tolist())

@@ -43,7 +43,10 @@ CKPT_BUDGET_BYTES = 16 << 30  # checkpoint memory reserved from max_steps alone
 class TraceSettings:
     """trace_rays keyword arguments (tracer.py:50-61) plus the checkpoint
     interval K, the checkpoint capacity per ray (None: enough for
-    max_steps) and the scoring sphere of captured rays, r = score_factor r_+."""
+    max_steps), the scoring sphere of captured rays, r = score_factor r_+,
+    an optional affine-parameter end, and whether the backward hands rays to
+    wavefront lanes in order of decreasing step count (same results; fewer
+    idle lanes)."""
 
     escape_radius: float
     max_steps: int = 20000
@@ -59,6 +62,7 @@ class TraceSettings:
     cap: int | None = None
     score_factor: float = 3.0
     lambda_end: float = math.inf
+    sort_backward: bool = True
 
 
 class Recorded:
@@ -137,12 +141,20 @@ class Recorded:
         gy0 = torch.empty((8, ld), dtype=torch.float64, device=dev)
         gspin = torch.empty(ld, dtype=torch.float64, device=dev)
         per_ray = lib.bh_adjoint_scratch_doubles(self.cfg.K)
-        scratch = torch.empty((per_ray, ld), dtype=torch.float64, device=dev)
+        if per_ray < 0:
+            raise ValueError("checkpoint interval K out of range")
+        # one block of per_ray x 64 doubles per wavefront of 64 rays
+        scratch = torch.empty(per_ray * (-(-n // 64) * 64 or 64), dtype=torch.float64,
+                              device=dev)
+        # longest reversals first, rays of similar length in one wavefront
+        order = torch.argsort(self.ev_step, descending=True) if self.cfg.sort_backward and n \
+            else None
         _lib.check(lib.bh_trace_adjoint_f64(
             self.params, n, self.cfg.K, self.score_radius, _lib.ptr(self.ckpt),
             _lib.ptr(self.event), _lib.ptr(self.ev_step), _lib.ptr(self.evy1),
             _lib.ptr(self.prev), _lib.ptr(self.theta), _lib.ptr(gterm), _lib.ptr(gy0),
-            _lib.ptr(gspin), _lib.ptr(scratch), _lib.stream_handle()), "trace_adjoint")
+            _lib.ptr(gspin), _lib.ptr(scratch), _lib.ptr(order), _lib.stream_handle()),
+            "trace_adjoint")
         return gy0[:, :n].t().contiguous(), gspin[:n]
 
 

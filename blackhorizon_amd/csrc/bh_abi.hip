@@ -181,7 +181,13 @@ int32_t bh_trace_batched_f64(const bh_trace_params *jobs, const int64_t *ray_off
   return hip_status(e, "bh_trace_batched_f64");
 }
 
-int32_t bh_adjoint_scratch_doubles(int32_t K) { return K < 1 ? -1 : (int32_t)adjoint_scratch_per_ray(K); }
+// Checkpoint intervals beyond this are refused (the per-ray scratch count
+// must fit int32).
+constexpr int32_t MAX_ADJOINT_K = 1 << 20;
+
+int32_t bh_adjoint_scratch_doubles(int32_t K) {
+  return K < 1 || K > MAX_ADJOINT_K ? -1 : (int32_t)adjoint_scratch_per_ray(K);
+}
 
 int32_t bh_trace_record_f64(const bh_trace_params *p, const double *state_in_soa, int64_t ld_in,
                             double *state_out_soa, int64_t ld_out, int32_t *status,
@@ -192,7 +198,8 @@ int32_t bh_trace_record_f64(const bh_trace_params *p, const double *state_in_soa
   int32_t rc = check_params(p);
   if (rc) return rc;
   if (n < 0 || ld_in < n || ld_out < n) return fail(BH_EINVAL, "bad n / ld");
-  if (K < 1 || cap < 1) return fail(BH_EINVAL, "K and cap must be >= 1");
+  if (K < 1 || K > MAX_ADJOINT_K || cap < 1)
+    return fail(BH_EINVAL, "need 1 <= K <= 2^20 and cap >= 1");
   if (n > 0 && (!state_in_soa || !state_out_soa || !status || !steps || !attempts || !ckpt ||
                 !n_ckpt || !term || !prev || !evy1 || !ev_step || !theta || !event))
     return fail(BH_EINVAL, "NULL buffer");
@@ -213,17 +220,17 @@ int32_t bh_trace_adjoint_f64(const bh_trace_params *p, int64_t n, int32_t K, dou
                              const double *ckpt, const int32_t *event, const int64_t *ev_step,
                              const double *evy1, const double *prev, const double *theta,
                              const double *gterm, double *gy0, double *gspin, double *scratch,
-                             void *stream) {
+                             const int64_t *order, void *stream) {
   int32_t rc = check_params(p);
   if (rc) return rc;
   if (n < 0) return fail(BH_EINVAL, "bad n");
-  if (K < 1) return fail(BH_EINVAL, "K must be >= 1");
+  if (K < 1 || K > MAX_ADJOINT_K) return fail(BH_EINVAL, "need 1 <= K <= 2^20");
   if (n > 0 && (!ckpt || !event || !ev_step || !evy1 || !prev || !theta || !gterm || !gy0 ||
                 !gspin || !scratch))
     return fail(BH_EINVAL, "NULL buffer");
   JobConst c = make_job_const(*p);
-  AdjointIO a{ckpt, event, ev_step, evy1, prev, theta, gterm, gy0, gspin, scratch, score_radius,
-              n, K};
+  AdjointIO a{ckpt,  event, ev_step, evy1,  prev,         theta, gterm, gy0,
+              gspin, scratch, order, score_radius, n, K};
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = p->mode == BH_MODE_PRECISE ? precise::launch_trace_adjoint(c, a, s)
                                             : fast::launch_trace_adjoint(c, a, s);

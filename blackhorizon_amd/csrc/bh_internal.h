@@ -88,15 +88,16 @@ struct AdjointIO {
   const double *gterm;  // [8][n] adjoint of the terminal point
   double *gy0;          // [8][n] adjoint of the initial state
   double *gspin;        // [n] per-ray spin adjoint
-  double *scratch;      // [9 K + 40][n]
+  double *scratch;      // [ceil(n / 64)][9 K][64]
+  const int64_t *order; // optional [n]: lane slot -> ray
   double score_radius;
   int64_t n;
   int32_t K;
 };
 
-// Doubles of adjoint scratch per ray: K replayed states + step sizes and
-// five parked DP5 stages.
-constexpr int64_t adjoint_scratch_per_ray(int64_t K) { return 9 * K + 40; }
+// Doubles of adjoint scratch per ray: K replayed steps (state and step size
+// each).
+__host__ __device__ constexpr int64_t adjoint_scratch_per_ray(int64_t K) { return 9 * K; }
 
 // Jobs one batched launch can hold (job table and ray offsets live in LDS).
 constexpr int MAX_BATCH_JOBS = 256;

@@ -140,7 +140,7 @@ int32_t bh_trace_batched_f64(const bh_trace_params *jobs, const int64_t *ray_off
  * trace_rays (tracer.py:50-157) to gradients of each ray's terminal point
  * with respect to its initial state and the spin. */
 
-/* Doubles of adjoint scratch per ray for checkpoint interval K (9 K + 40),
+/* Doubles of adjoint scratch per ray for checkpoint interval K (9 K),
  * or -1 for K < 1. */
 int32_t bh_adjoint_scratch_doubles(int32_t K);
 
@@ -167,13 +167,16 @@ int32_t bh_trace_record_f64(const bh_trace_params *p, const double *state_in_soa
  * gy0 [8][n] = dL/d(initial state) and gspin [n] = per-ray dL/d(spin),
  * differentiating every accepted step with its step size held fixed (each
  * checkpoint segment is replayed through the controller).  scratch:
- * [bh_adjoint_scratch_doubles(K)][n] device doubles.  Rays with event -1
- * get NaN. */
+ * bh_adjoint_scratch_doubles(K) * (n rounded up to a multiple of 64) device
+ * doubles (opaque; laid out per wavefront of 64 rays).  order: optional
+ * device int64[n] permutation, lane slot k reverses ray order[k] (NULL =
+ * identity); ordering rays by ev_step keeps a wavefront's lanes busy for
+ * the same number of steps.  Rays with event -1 get NaN. */
 int32_t bh_trace_adjoint_f64(const bh_trace_params *p, int64_t n, int32_t K, double score_radius,
                              const double *ckpt, const int32_t *event, const int64_t *ev_step,
                              const double *evy1, const double *prev, const double *theta,
                              const double *gterm, double *gy0, double *gspin, double *scratch,
-                             void *stream);
+                             const int64_t *order, void *stream);
 
 /* Vector-Jacobian product of geodesic_rhs (geodesics.py:42-74) at n states
  * (rows): zbar = J^T v (n, 8) and abar = v . d f / d spin (n). */
