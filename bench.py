@@ -5,10 +5,11 @@
 One bench step traces the whole 1920x1080 sweep -- spins a in {0, 0.5, 0.9,
 0.998} x inclinations {17, 80} deg, camera at r = 50 M, horizontal FOV 25 deg,
 rtol 1e-8 / atol 1e-10, capture at r_+ + 0.01 M, escape at 100 M, max_steps
-20000 -- one plain-tracer launch per frame (8 launches, 16.6 M rays).  Rays
-are generated on the device before timing (camera directions on the host,
-null momenta and the SoA layout on the device), so the timed region is the
-tracer kernels only, with inputs resident in HBM.
+20000 -- one plain-tracer launch per frame (8 launches, 16.6 M rays), rays
+handed to wavefront lanes in 8x8 pixel tiles (--order; results do not depend
+on it).  Rays are generated on the device before timing (camera directions
+on the host, null momenta and the SoA layout on the device), so the timed
+region is the tracer kernels only, with inputs resident in HBM.
 
 Prints one JSON line.  value = accepted ray-steps per second over the sweep.
 roofline: algorithmic FP64 FLOP of the tracer kernel (1321 FLOP per DP5
@@ -16,7 +17,9 @@ attempt: 6 RHS x 146 + 406 stage + 36 error-norm + 3 controller) per launch,
 over the kernel's mean launch time measured with HIP events on its stream,
 against the FP64 vector peak (78.6 TFLOP/s).  cpu_baseline: the NumPy oracle
 (a restatement of the reference tracer) on a seeded subsample of every frame,
-on this host's cores, timed before the GPU is touched.
+on this host's cores, timed before the GPU is touched.  adjoint_over_forward:
+the discrete-adjoint op's time over the plain forward trace's on the 512^2
+shadow fit (--workload shadowfit), measured after the sweep.
 """
 
 from __future__ import annotations
@@ -35,6 +38,10 @@ if ROOT not in sys.path:
 
 FLOP_PER_ATTEMPT = 1321
 FP64_VECTOR_PEAK_TFLOPS = 78.6
+# HBM bytes per trace_plain_kernel launch on the 1920x1080 frames (fast
+# build): rocprofv3 FETCH_SIZE + WRITE_SIZE, each calibrated on the layout
+# copy aos_to_soa_kernel of the same run (profiles/trace_plain_hd_sweep.md).
+HD_TRAFFIC_BYTES_PER_LAUNCH = 3.007e8
 METRIC = "ray-steps/s FP64 @rtol 1e-8, % FP64 vector peak, adjoint/forward time"
 
 
@@ -424,10 +431,12 @@ def main(argv=None) -> int:
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--rays-per-lane", type=int, default=1)
     ap.add_argument("--refill-min", type=int, default=1)
-    ap.add_argument("--order", default="row",
+    ap.add_argument("--order", default="tile8",
                     help="ray hand-out order: row (pixel order) or tile<k> (k x k pixel tiles)")
     ap.add_argument("--cpu-rays-per-frame", type=int, default=6144)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="hd/hd09: skip the adjoint/forward measurement (512^2 shadow fit)")
     args = ap.parse_args(argv)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -512,6 +521,13 @@ def main(argv=None) -> int:
                               status_counts=st[:7]))
         steps_sweep += s
         attempts_sweep += a
+    adj = None
+    if not args.no_adjoint and args.workload in ("hd", "hd09"):
+        # the metric's third part: adjoint op time over plain forward time on
+        # the 512^2 shadow fit (a few seconds, outside the timed sweep)
+        fit_args = argparse.Namespace(**{**vars(args), "steps": 3, "warmup": 1})
+        adj = run_shadowfit(fit_args)
+        print("[bench] adjoint/forward measured", file=sys.stderr, flush=True)
     total_kernel_s = sum(kern_ms) * 1e-3
     value = steps_sweep * args.steps / wall
     flop = attempts_sweep * args.steps * FLOP_PER_ATTEMPT
@@ -547,14 +563,19 @@ def main(argv=None) -> int:
         "ns_per_ray_step": wall / (steps_sweep * args.steps) * 1e9,
         "attempts_per_s": attempts_sweep * args.steps / wall,
         "pct_fp64_vector_peak": 100.0 * achieved_tflops / FP64_VECTOR_PEAK_TFLOPS,
-        "adjoint_over_forward": None,
+        "adjoint_over_forward": None if adj is None else adj["adjoint_over_forward"],
+        "adjoint": None if adj is None else {
+            k: adj[k] for k in ("rays", "ray_steps", "ms_plain_forward", "ms_recording_forward",
+                                "ms_adjoint", "ms_fit_step", "adjoint_over_forward")},
         "roofline": {
             "bound": "valu",
             "achieved": achieved_tflops,
             "peak": FP64_VECTOR_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP64_VECTOR_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": HD_TRAFFIC_BYTES_PER_LAUNCH if (
+                args.workload == "hd"
+                and tracer.resolve_mode(args.mode, jobs[0].rtol) == "fast") else None,
             "kernel": "trace_plain_kernel ("
                       + tracer.resolve_mode(args.mode, jobs[0].rtol) + " build)",
             "flop_per_launch": flop / len(events),

@@ -14,7 +14,9 @@ import re
 
 import torch  # noqa: F401  (must precede loading the library: HIP runtime binding)
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libbhtrace.so")
+# BH_TRACE_LIB selects another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("BH_TRACE_LIB") or os.path.join(
+    os.path.dirname(os.path.abspath(__file__)), "lib", "libbhtrace.so")
 HEADER_PATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                            "include", "bh_trace.h")
 

@@ -2,6 +2,13 @@
 
     python -m blackhorizon_amd.profiling stats <results.db> [--top N]
     python -m blackhorizon_amd.profiling pmc <results.db> [--match trace]
+    python -m blackhorizon_amd.profiling traffic <fetch.db> --write-db <write.db> --match trace
+
+``traffic`` turns FETCH_SIZE and WRITE_SIZE (KiB per dispatch, from two
+separate ``--pmc`` runs of the same command) into HBM bytes per dispatch of
+the matching kernels, each counter scaled by what it reads for the layout
+copy ``aos_to_soa_kernel`` in the same run: that kernel reads and writes
+exactly 64 bytes per thread of its grid (n a multiple of 256).
 
 ``stats`` prints per-kernel dispatch counts and durations in ms (the
 ``--kernel-trace --stats`` view).  ``pmc`` prints, per dispatch of the
@@ -48,13 +55,56 @@ def pmc_by_dispatch(db: str, match: str = "") -> list[dict]:
     return list(out.values())
 
 
+CALIB_KERNEL = "aos_to_soa_kernel"
+CALIB_BYTES_PER_THREAD = 64
+
+
+def _calibration(rows: list[dict], counter: str) -> float:
+    """Known bytes / counted bytes of the layout copies in one run."""
+    ratios = [d["grid"] * CALIB_BYTES_PER_THREAD / (d["counters"][counter] * 1024.0)
+              for d in rows if CALIB_KERNEL in d["kernel"] and d["counters"].get(counter)]
+    if not ratios:
+        raise SystemExit(f"no {CALIB_KERNEL} dispatch with {counter} in this run")
+    return sum(ratios) / len(ratios)
+
+
+def traffic(fetch_db: str, write_db: str, match: str) -> dict:
+    """Calibrated HBM read/write bytes per dispatch of the matching kernels
+    (paired by order of appearance in the two runs)."""
+    f_rows = pmc_by_dispatch(fetch_db)
+    w_rows = pmc_by_dispatch(write_db)
+    kf = _calibration(f_rows, "FETCH_SIZE")
+    kw = _calibration(w_rows, "WRITE_SIZE")
+    fm = [d for d in f_rows if match in d["kernel"]]
+    wm = [d for d in w_rows if match in d["kernel"]]
+    if len(fm) != len(wm) or not fm:
+        raise SystemExit(f"{len(fm)} FETCH vs {len(wm)} WRITE dispatches match {match!r}")
+    per = []
+    for a, b in zip(fm, wm):
+        rd = a["counters"]["FETCH_SIZE"] * 1024.0 * kf
+        wr = b["counters"]["WRITE_SIZE"] * 1024.0 * kw
+        per.append(dict(kernel=a["kernel"], grid=a["grid"], read_bytes=rd, write_bytes=wr))
+    mean = sum(p["read_bytes"] + p["write_bytes"] for p in per) / len(per)
+    return dict(fetch_factor=kf, write_factor=kw, dispatches=per, mean_bytes_per_dispatch=mean)
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("what", choices=["stats", "pmc"])
+    ap.add_argument("what", choices=["stats", "pmc", "traffic"])
     ap.add_argument("db")
     ap.add_argument("--top", type=int, default=12)
     ap.add_argument("--match", default="")
+    ap.add_argument("--write-db", help="traffic: the WRITE_SIZE run's database")
     a = ap.parse_args(argv)
+    if a.what == "traffic":
+        t = traffic(a.db, a.write_db, a.match)
+        print(f"calibration ({CALIB_KERNEL}, {CALIB_BYTES_PER_THREAD} B/thread each way): "
+              f"FETCH_SIZE x {t['fetch_factor']:.4f}, WRITE_SIZE x {t['write_factor']:.4f}")
+        for d in t["dispatches"]:
+            print(f"{d['kernel'][:70]} grid={d['grid']} read={d['read_bytes'] / 1e6:.3f} MB "
+                  f"write={d['write_bytes'] / 1e6:.3f} MB")
+        print(f"mean HBM bytes per dispatch: {t['mean_bytes_per_dispatch']:.6g}")
+        return 0
     if a.what == "stats":
         print(f"{'kernel':<90} {'calls':>6} {'total_ms':>10} {'avg_ms':>9} {'pct':>6}")
         for name, calls, tot, avg, pct in kernel_stats(a.db, a.top):
