@@ -146,41 +146,32 @@ def device_state(job):
 
 
 def prepare_gpu(jobs, mode, order="row"):
+    """One device-resident TracePlan per job (rays built on the device)."""
     import torch
 
-    from blackhorizon_amd import _lib, tracer
+    from blackhorizon_amd import tracer
 
-    lib = _lib.require_device()
-    dev = torch.device("cuda", 0)
-    stream = _lib.stream_handle()
     prepared = []
     for job in jobs:
         st, state = device_state(job)
-        n = job.n_rays
-        soa_in = torch.empty((8, n), dtype=torch.float64, device=dev)
-        _lib.check(lib.bh_aos_to_soa_f64(_lib.ptr(state), _lib.ptr(soa_in), n, n, stream), "soa")
-        del state
-        params = tracer.make_params(st, job.escape_radius, job.max_steps, job.rtol, job.atol,
-                                    job.initial_step, job.max_step, job.min_step,
-                                    job.capture_margin, mode)
         o = camera_order(job, order)
-        bufs = dict(soa_in=soa_in, soa_out=torch.empty_like(soa_in),
-                    status=torch.empty(n, dtype=torch.int32, device=dev),
-                    steps=torch.empty(n, dtype=torch.int64, device=dev),
-                    attempts=torch.empty(n, dtype=torch.int64, device=dev),
-                    iters=torch.zeros(1, dtype=torch.int64, device=dev),
-                    order=None if o is None else torch.from_numpy(o).to(dev))
-        prepared.append((job, params, bufs))
+        plan = tracer.TracePlan(st, state, job.escape_radius, job.max_steps, job.rtol, job.atol,
+                                job.initial_step, job.max_step, job.min_step,
+                                job.capture_margin, mode=mode,
+                                ray_order=None if o is None else torch.from_numpy(o))
+        plan.state0 = None  # the SoA copy is all the kernel reads
+        del state
+        prepared.append((job, plan))
     torch.cuda.synchronize()
     return prepared
 
 
-def lane_efficiency(b, rays_per_lane):
+def lane_efficiency(plan, rays_per_lane):
     """Fraction of lane-attempts doing useful work when every 64 consecutive
     rays of the hand-out order share a wavefront (exact for rays_per_lane 1)."""
     import torch
 
-    a = b["attempts"] if b.get("order") is None else b["attempts"][b["order"]]
+    a = plan.attempts if plan.order is None else plan.attempts[plan.order]
     n = a.shape[0]
     pad = (-n) % 64
     if pad:
@@ -189,15 +180,24 @@ def lane_efficiency(b, rays_per_lane):
     return float(a.sum().item()) / float((64 * w.max(dim=1).values).sum().item())
 
 
-def launch(lib, params, tune, b, stream):
-    from blackhorizon_amd import _lib
+def pick_tune(args, prepared, kind_key):
+    """--tune: "auto" runs the autotuner on the first plan (its table goes
+    into the JSON line), "default" is (256, 1, 1), "B,R,M" is explicit."""
+    from blackhorizon_amd import autotune, tracer
 
-    n = b["soa_in"].shape[1]
-    _lib.check(lib.bh_trace_plain_f64(params, tune, _lib.ptr(b["soa_in"]), n,
-                                      _lib.ptr(b["soa_out"]), n, _lib.ptr(b["status"]),
-                                      _lib.ptr(b["steps"]), _lib.ptr(b["attempts"]),
-                                      _lib.ptr(b["iters"]), _lib.ptr(b.get("order")), n, stream),
-               "trace")
+    if args.tune == "default":
+        return tracer.Tune(), None
+    if args.tune != "auto":
+        b, r, m = (int(v) for v in args.tune.split(","))
+        return tracer.Tune(b, r, m), None
+    plan = prepared[0][1]
+    res = autotune.autotune(plan, workload_key=kind_key, cache=None, reps=2)
+    print(f"[bench] autotune: {res.best} ({res.best_ms:.3f} ms)", file=sys.stderr, flush=True)
+    info = dict(autotuned_on=kind_key, best_ms=res.best_ms,
+                candidates=len(res.table),
+                fastest5=sorted(([t, round(ms, 4)] for t, ms in res.table),
+                                key=lambda e: e[1])[:5])
+    return res.best, info
 
 
 def run_uneven(args) -> dict:
@@ -206,9 +206,9 @@ def run_uneven(args) -> dict:
     launch per job on one stream.  Same rays, same per-ray results."""
     import torch
 
-    from blackhorizon_amd import _lib, batch, tracer, workloads
+    from blackhorizon_amd import _lib, autotune, batch, graph, tracer, workloads
 
-    lib = _lib.require_device()
+    _lib.require_device()
     jobs = workloads.uneven_mix()
     tjobs = []
     for job in jobs:
@@ -216,15 +216,20 @@ def run_uneven(args) -> dict:
         tjobs.append(batch.TraceJob(st, state, job.escape_radius, job.max_steps, job.rtol,
                                     job.atol, job.initial_step, job.max_step, job.min_step,
                                     job.capture_margin))
-    tune = tracer.Tune(args.block, 1, args.refill_min)
-    plan = batch.BatchPlan(tjobs, mode=args.mode, tune=tune, concurrent=not args.serial_groups)
+    plan = batch.BatchPlan(tjobs, mode=args.mode, concurrent=not args.serial_groups)
     prepared = prepare_gpu(jobs, args.mode, "row")
     for t in tjobs:
         t.state0 = None
     torch.cuda.synchronize()
     cur = torch.cuda.current_stream()
-    stream = _lib.stream_handle()
-    tc = tune.as_c()
+    tune_info = None
+    if args.tune == "auto":
+        res = autotune.autotune(plan, workload_key="uneven64", cache=None, reps=2)
+        tune_info = dict(best=res.best.__dict__, best_ms=res.best_ms, candidates=len(res.table))
+        print(f"[bench] autotune batched: {res.best}", file=sys.stderr, flush=True)
+    elif args.tune != "default":
+        b_, _, m_ = (int(v) for v in args.tune.split(","))
+        plan.tune = tracer.Tune(b_, 1, m_)
 
     def timed(fn, k):
         e0 = torch.cuda.Event(enable_timing=True)
@@ -237,22 +242,32 @@ def run_uneven(args) -> dict:
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / k
 
-    def per_job():
-        for job, params, b in prepared:
-            launch(lib, params, tc, b, stream)
+    class PerJob:
+        """One plain launch per job, in job order on one stream."""
 
+        def launch(self, tune=None):
+            for _, p in prepared:
+                p.launch()
+
+    per_job = PerJob()
     for _ in range(args.warmup):
         plan.launch()
-        per_job()
+        per_job.launch()
     torch.cuda.synchronize()
     ms_batched = timed(plan.launch, args.steps)
     print("[bench] batched done", file=sys.stderr, flush=True)
-    ms_per_job = timed(per_job, args.steps)
+    ms_per_job = timed(per_job.launch, args.steps)
     print("[bench] per-job done", file=sys.stderr, flush=True)
+    # the same launch sequences captured in HIP graphs
+    g_batched = graph.TraceGraph(plan)
+    g_per_job = graph.TraceGraph(per_job)
+    ms_batched_graph = timed(g_batched.replay, args.steps)
+    ms_per_job_graph = timed(g_per_job.replay, args.steps)
+    print("[bench] graph replays done", file=sys.stderr, flush=True)
     steps_b = sum(int(g.steps.sum().item()) for g in plan.groups)
     att_b = sum(int(g.attempts.sum().item()) for g in plan.groups)
-    steps_p = sum(int(b["steps"].sum().item()) for _, _, b in prepared)
-    same = all(torch.equal(g.steps, torch.cat([prepared[i][2]["steps"] for i in g.idx]))
+    steps_p = sum(int(p.steps.sum().item()) for _, p in prepared)
+    same = all(torch.equal(g.steps, torch.cat([prepared[i][1].steps for i in g.idx]))
                for g in plan.groups)
     modes = {g.mode: len(g.idx) for g in plan.groups}
     return dict(
@@ -260,6 +275,9 @@ def run_uneven(args) -> dict:
         per_job_ray_steps_per_s=steps_p / (ms_per_job * 1e-3),
         speedup=ms_per_job / ms_batched,
         ms_batched=ms_batched, ms_per_job=ms_per_job,
+        ms_batched_graph=ms_batched_graph, ms_per_job_graph=ms_per_job_graph,
+        speedup_graph=ms_per_job_graph / ms_batched_graph,
+        tune=tune_info or plan.tune.__dict__,
         ray_steps=steps_b, attempts=att_b, rays=plan.total_rays,
         pct_fp64_vector_peak_batched=100.0 * att_b * FLOP_PER_ATTEMPT / (ms_batched * 1e-3)
         / (FP64_VECTOR_PEAK_TFLOPS * 1e12),
@@ -428,9 +446,9 @@ def main(argv=None) -> int:
     ap.add_argument("--serial-groups", action="store_true",
                     help="uneven: run the fast and precise batches one after the other")
     ap.add_argument("--mode", default="auto", choices=["auto", "fast", "precise"])
-    ap.add_argument("--block", type=int, default=256)
-    ap.add_argument("--rays-per-lane", type=int, default=1)
-    ap.add_argument("--refill-min", type=int, default=1)
+    ap.add_argument("--tune", default="auto",
+                    help='launch shape: "auto" (autotuner, untimed, before the sweep), '
+                         '"default" (256,1,1) or "block,rays_per_lane,refill_min"')
     ap.add_argument("--order", default="tile8",
                     help="ray hand-out order: row (pixel order) or tile<k> (k x k pixel tiles)")
     ap.add_argument("--cpu-rays-per-frame", type=int, default=6144)
@@ -476,15 +494,16 @@ def main(argv=None) -> int:
 
     from blackhorizon_amd import _lib, tracer
 
-    lib = _lib.require_device()
+    _lib.require_device()
     prepared = prepare_gpu(jobs, args.mode, args.order)
-    tune = tracer.Tune(args.block, args.rays_per_lane, args.refill_min).as_c()
-    stream = _lib.stream_handle()
+    # tune on the a = 0.9, 17 deg frame when present (else the first)
+    tune_plans = [jp for jp in prepared if jp[0].spin == 0.9] or prepared
+    tune, tune_info = pick_tune(args, tune_plans, f"{args.workload}-a0.9-{args.order}")
     cur = torch.cuda.current_stream()
 
     for w in range(args.warmup):
-        for job, params, b in prepared:
-            launch(lib, params, tune, b, stream)
+        for job, plan in prepared:
+            plan.launch(tune)
         torch.cuda.synchronize()
         print(f"[bench] warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
 
@@ -492,11 +511,11 @@ def main(argv=None) -> int:
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        for job, params, b in prepared:
+        for job, plan in prepared:
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record(cur)
-            launch(lib, params, tune, b, stream)
+            plan.launch(tune)
             e1.record(cur)
             events.append((e0, e1))
     torch.cuda.synchronize()
@@ -505,15 +524,15 @@ def main(argv=None) -> int:
     kern_ms = [a.elapsed_time(b) for a, b in events]
     per_frame = []
     steps_sweep = attempts_sweep = 0
-    for i, (job, params, b) in enumerate(prepared):
-        s = int(b["steps"].sum().item())
-        a = int(b["attempts"].sum().item())
-        st = torch.bincount(b["status"].long(), minlength=7).tolist()
+    for i, (job, plan) in enumerate(prepared):
+        s = int(plan.steps.sum().item())
+        a = int(plan.attempts.sum().item())
+        st = torch.bincount(plan.status.long(), minlength=7).tolist()
         ms = [kern_ms[k * len(prepared) + i] for k in range(args.steps)]
         per_frame.append(dict(spin=job.spin, inclination=job.inclination_degrees,
-                              lane_efficiency=round(lane_efficiency(b, args.rays_per_lane), 4),
+                              lane_efficiency=round(lane_efficiency(plan, tune.rays_per_lane), 4),
                               rays=job.n_rays, ray_steps=s, attempts=a,
-                              iterations=int(b["iters"].item()),
+                              iterations=int(plan.iters.item()),
                               kernel_ms=round(float(np.mean(ms)), 3),
                               ns_per_ray_step=float(np.mean(ms)) * 1e6 / max(s, 1),
                               pct_fp64_peak=100.0 * a * FLOP_PER_ATTEMPT
@@ -556,8 +575,8 @@ def main(argv=None) -> int:
             "rays_per_step": int(sum(j.n_rays for j in jobs)),
             "launches_per_step": len(jobs),
             "mode": args.mode,
-            "tune": {"block": args.block, "rays_per_lane": args.rays_per_lane,
-                     "refill_min": args.refill_min},
+            "tune": {"block": tune.block_size, "rays_per_lane": tune.rays_per_lane,
+                     "refill_min": tune.refill_min, "autotune": tune_info},
             "order": args.order,
         },
         "ns_per_ray_step": wall / (steps_sweep * args.steps) * 1e9,

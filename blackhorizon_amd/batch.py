@@ -127,7 +127,7 @@ class BatchPlan:
             raise ValueError("no jobs")
         dev = _dev.device()
         self.jobs = jobs
-        self.tune = (tune or Tune()).as_c()
+        self.tune = tune or Tune()
         by_mode: dict[str, list[int]] = {}
         for i, j in enumerate(jobs):
             by_mode.setdefault(resolve_mode(mode, j.rtol), []).append(i)
@@ -139,18 +139,20 @@ class BatchPlan:
         if concurrent and len(self.groups) > 1:
             self.streams = [torch.cuda.Stream(device=dev) for _ in self.groups]
 
-    def launch(self) -> None:
-        """Enqueue every group (no host synchronisation)."""
+    def launch(self, tune: Tune | None = None) -> None:
+        """Enqueue every group (no host synchronisation); ``tune`` overrides
+        the plan's launch shape (results do not depend on it)."""
+        t = (tune or self.tune).as_c()
         cur = torch.cuda.current_stream()
         if self.streams is None:
             for g in self.groups:
-                g.launch(self.tune, _lib.stream_handle(cur))
+                g.launch(t, _lib.stream_handle(cur))
             return
         ev = torch.cuda.Event()
         ev.record(cur)
         for g, s in zip(self.groups, self.streams):
             s.wait_event(ev)
-            g.launch(self.tune, _lib.stream_handle(s))
+            g.launch(t, _lib.stream_handle(s))
         for s in self.streams:
             cur.wait_stream(s)
 

@@ -94,6 +94,76 @@ def _order_tensor(ray_order, n: int, dev):
     return o
 
 
+class TracePlan:
+    """Device-resident inputs, outputs and parameters of one plain trace.
+
+    The constructor uploads ``state0`` (SoA), checks the initial radii
+    (tracer.py:94-99) and allocates the outputs; ``launch()`` only enqueues
+    the tracer kernel on the current stream -- no allocation, no host
+    synchronisation, no memset -- so it can be repeated, timed or captured in
+    a HIP graph (``graph.TraceGraph``); ``result()`` converts the outputs to
+    a TraceResult.  Arguments are trace_rays'.
+    """
+
+    def __init__(self, spacetime: KerrSpacetime, state0: Array, escape_radius: float,
+                 max_steps: int = 20000, rtol: float = 1e-9, atol: float = 1e-12,
+                 initial_step: float = 0.1, max_step: float = 2.0, min_step: float = 1e-10,
+                 capture_margin: float = 1e-3, *, mode: str = "auto", tune: Tune | None = None,
+                 ray_order: Array | None = None, lambda_end: float = math.inf):
+        self.params = make_params(spacetime, escape_radius, max_steps, rtol, atol, initial_step,
+                                  max_step, min_step, capture_margin, mode, lambda_end)
+        self.mode = resolve_mode(mode, rtol)
+        self.state0 = state0
+        s0 = _dev.as_device_f64(state0, (8,), "state0")
+        n = s0.shape[0]
+        self.n = n
+        dev = s0.device
+        lib = _lib.require_device()
+        stream = _lib.stream_handle()
+        self.order = _order_tensor(ray_order, n, dev)
+        ld = max(n, 1)
+        self.soa_in = torch.empty((8, ld), dtype=torch.float64, device=dev)
+        _lib.check(lib.bh_aos_to_soa_f64(_lib.ptr(s0), _lib.ptr(self.soa_in), ld, n, stream),
+                   "aos_to_soa")
+        count = torch.zeros(1, dtype=torch.int32, device=dev)
+        _lib.check(lib.bh_count_outside_f64(self.params, _lib.ptr(self.soa_in), ld, n,
+                                            _lib.ptr(count), stream), "count_outside")
+        if int(count.item()) > 0:
+            raise ValueError("escape_radius must exceed all initial radii")
+        self.soa_out = torch.empty_like(self.soa_in)
+        self.status = torch.empty(n, dtype=torch.int32, device=dev)
+        self.steps = torch.empty(n, dtype=torch.int64, device=dev)
+        self.attempts = torch.empty(n, dtype=torch.int64, device=dev)
+        self.iters = torch.zeros(1, dtype=torch.int64, device=dev)
+        self.tune = tune or Tune()
+
+    def launch(self, tune: Tune | None = None) -> None:
+        """Enqueue the trace on the current stream (``tune`` overrides the
+        plan's launch shape; results do not depend on it)."""
+        lib = _lib.load()
+        ld = self.soa_in.shape[1]
+        _lib.check(lib.bh_trace_plain_f64(
+            self.params, (tune or self.tune).as_c(), _lib.ptr(self.soa_in), ld,
+            _lib.ptr(self.soa_out), ld, _lib.ptr(self.status), _lib.ptr(self.steps),
+            _lib.ptr(self.attempts), _lib.ptr(self.iters), _lib.ptr(self.order), self.n,
+            _lib.stream_handle()), "trace_rays")
+
+    def result(self) -> TraceResult:
+        """The last launch's outputs, NumPy or device tensors like state0."""
+        lib = _lib.load()
+        n = self.n
+        states = torch.empty((n, 8), dtype=torch.float64, device=self.soa_out.device)
+        _lib.check(lib.bh_soa_to_aos_f64(_lib.ptr(self.soa_out), self.soa_out.shape[1],
+                                         _lib.ptr(states), n, _lib.stream_handle()),
+                   "soa_to_aos")
+        ref = self.state0
+        return TraceResult(states=_dev.like_input(states, ref),
+                           status=_dev.like_input(self.status.clone(), ref),
+                           steps=_dev.like_input(self.steps.clone(), ref),
+                           iterations=int(self.iters.item()),
+                           attempts=_dev.like_input(self.attempts.clone(), ref))
+
+
 def trace_rays(
     spacetime: KerrSpacetime,
     state0: Array,
@@ -126,41 +196,11 @@ def trace_rays(
     ``lambda_end`` (affine parameter) ends rays that reach it with status
     TERMINATED (see integrate_rays).
     """
-    params = make_params(spacetime, escape_radius, max_steps, rtol, atol, initial_step,
-                         max_step, min_step, capture_margin, mode, lambda_end)
-    s0 = _dev.as_device_f64(state0, (8,), "state0")
-    n = s0.shape[0]
-    dev = s0.device
-    lib = _lib.require_device()
-    stream = _lib.stream_handle()
-    order = _order_tensor(ray_order, n, dev)
-    soa_in = torch.empty((8, max(n, 1)), dtype=torch.float64, device=dev)
-    _lib.check(lib.bh_aos_to_soa_f64(_lib.ptr(s0), _lib.ptr(soa_in), soa_in.shape[1], n, stream),
-               "aos_to_soa")
-    count = torch.zeros(1, dtype=torch.int32, device=dev)
-    _lib.check(lib.bh_count_outside_f64(params, _lib.ptr(soa_in), soa_in.shape[1], n,
-                                        _lib.ptr(count), stream), "count_outside")
-    if int(count.item()) > 0:
-        raise ValueError("escape_radius must exceed all initial radii")
-    soa_out = torch.empty_like(soa_in)
-    status = torch.empty(n, dtype=torch.int32, device=dev)
-    steps = torch.empty(n, dtype=torch.int64, device=dev)
-    attempts = torch.empty(n, dtype=torch.int64, device=dev)
-    iters = torch.zeros(1, dtype=torch.int64, device=dev)
-    t = (tune or Tune()).as_c()
-    _lib.check(lib.bh_trace_plain_f64(params, t, _lib.ptr(soa_in), soa_in.shape[1],
-                                      _lib.ptr(soa_out), soa_out.shape[1], _lib.ptr(status),
-                                      _lib.ptr(steps), _lib.ptr(attempts), _lib.ptr(iters),
-                                      _lib.ptr(order), n, stream), "trace_rays")
-    states = torch.empty((n, 8), dtype=torch.float64, device=dev)
-    _lib.check(lib.bh_soa_to_aos_f64(_lib.ptr(soa_out), soa_out.shape[1], _lib.ptr(states), n,
-                                     stream), "soa_to_aos")
-    iterations = int(iters.item())
-    return TraceResult(states=_dev.like_input(states, state0),
-                       status=_dev.like_input(status, state0),
-                       steps=_dev.like_input(steps, state0),
-                       iterations=iterations,
-                       attempts=_dev.like_input(attempts, state0))
+    plan = TracePlan(spacetime, state0, escape_radius, max_steps, rtol, atol, initial_step,
+                     max_step, min_step, capture_margin, mode=mode, tune=tune,
+                     ray_order=ray_order, lambda_end=lambda_end)
+    plan.launch()
+    return plan.result()
 
 
 def integrate_rays(
