@@ -25,6 +25,9 @@ BH_EINVAL = -1
 BH_EHIP = -2
 MODE_FAST = 0
 MODE_PRECISE = 1
+DENSE_DONE = 0
+DENSE_CHUNK_FULL = 1
+DENSE_UNDERFLOW = 2
 ABI_VERSION = 1
 
 
@@ -84,6 +87,8 @@ SIGNATURES = {
     "bh_geometry_f64": [_D, _D, _I32, _P, _I64, _P, _P, _P, _P, _P, _P],
     "bh_geodesic_rhs_f64": [_D, _D, _I32, _P, _I64, _P, _P],
     "bh_dp5_step_f64": [_D, _D, _I32, _D, _D, _P, _P, _I64, _P, _P, _P, _P, _P],
+    "bh_integrate_dense_f64": [_D, _D, _I32, _D, _D, _D, _D, _P, _I64, _I64, _P, _P, _I64, _P,
+                               _P, _P, _P],
     "bh_null_momentum_f64": [_D, _D, _P, _P, _I64, _I32, _I32, _P, _P],
     "bh_timelike_momentum_f64": [_D, _D, _P, _P, _I64, _P, _P, _P],
     "bh_diagnostics_f64": [_D, _D, _P, _I64, _P, _P, _P, _P, _P],

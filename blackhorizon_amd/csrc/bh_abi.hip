@@ -3,6 +3,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <cmath>
+
 #include "bh_internal.h"
 
 namespace bh {
@@ -304,6 +306,29 @@ int32_t bh_dp5_step_f64(double mass, double spin, int32_t mode, double rtol, dou
           ? precise::launch_dp5_step(c, state, h, n, new_state, error, ratio, factor, s)
           : fast::launch_dp5_step(c, state, h, n, new_state, error, ratio, factor, s);
   return hip_status(e, "bh_dp5_step_f64");
+}
+
+int32_t bh_integrate_dense_f64(double mass, double spin, int32_t mode, double rtol,
+                               double atol, double max_step, double lambda_total,
+                               double *state_soa, int64_t ld, int64_t n, double *work,
+                               double *traj, int64_t cap, double *ctl, int64_t *count,
+                               int32_t *flag, void *stream) {
+  int32_t rc = check_spacetime(mass, spin);
+  if (rc) return rc;
+  if (mode != BH_MODE_FAST && mode != BH_MODE_PRECISE) return fail(BH_EINVAL, "bad mode");
+  if (n < 1 || ld < n || cap < 1) return fail(BH_EINVAL, "need n >= 1, ld >= n, cap >= 1");
+  if (!(rtol >= 0.0) || !(atol >= 0.0) || !(max_step > 0.0) || std::isnan(lambda_total))
+    return fail(BH_EINVAL, "bad tolerances, max_step or lambda_total");
+  if (!state_soa || !work || !traj || !ctl || !count || !flag)
+    return fail(BH_EINVAL, "NULL buffer");
+  JobConst c = spacetime_const(mass, spin);
+  c.rtol = rtol;
+  c.atol = atol;
+  DenseIO d{state_soa, ld, work, traj, ctl, count, flag, n, cap, lambda_total, max_step};
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = mode == BH_MODE_PRECISE ? precise::launch_integrate_dense(c, d, s)
+                                         : fast::launch_integrate_dense(c, d, s);
+  return hip_status(e, "bh_integrate_dense_f64");
 }
 
 int32_t bh_null_momentum_f64(double mass, double spin, const double *positions,

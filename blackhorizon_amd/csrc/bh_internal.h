@@ -95,6 +95,19 @@ struct AdjointIO {
   int32_t K;
 };
 
+// Batch-shared dense integration (integrators.py:127-174), one chunk.
+struct DenseIO {
+  double *y;            // [8][ld] in: states at the chunk's start; out: at its end
+  int64_t ld;
+  double *work;         // [24][ld] scratch
+  double *traj;         // [cap][8][n] accepted states of this chunk
+  double *ctl;          // [2] elapsed affine parameter, step size (in/out)
+  int64_t *count;       // [1] accepted states written
+  int32_t *flag;        // [1] BH_DENSE_*
+  int64_t n, cap;
+  double lambda_total, max_step;
+};
+
 // Doubles of adjoint scratch per ray: K replayed steps (state and step size
 // each).
 __host__ __device__ constexpr int64_t adjoint_scratch_per_ray(int64_t K) { return 9 * K; }
@@ -123,6 +136,7 @@ struct LaunchShape {
                                   hipStream_t s);                                  \
   hipError_t launch_rhs_vjp(const JobConst &c, const double *state, const double *v, \
                             int64_t n, double *zbar, double *abar, hipStream_t s); \
+  hipError_t launch_integrate_dense(const JobConst &c, const DenseIO &d, hipStream_t s); \
   hipError_t launch_count_outside(const JobConst &c, const double *in, int64_t ld, \
                                   int64_t n, int32_t *count, hipStream_t s);       \
   hipError_t launch_geometry(const JobConst &c, const double *xyz, int64_t n,      \

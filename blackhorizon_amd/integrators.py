@@ -81,3 +81,63 @@ def dormand_prince_attempt(rhs, state: Array, h: Array, rtol: float, atol: float
     and factor is step_factor(ratio) (integrators.py:120-124)."""
     out = _step(rhs, state, h, rtol, atol)
     return tuple(_dev.like_input(t, state) for t in out)
+
+
+def integrate_dense(rhs, state0: Array, lambda_total: float, rtol: float = 1e-10,
+                    atol: float = 1e-12, initial_step: float = 0.05, max_step: float = 2.0,
+                    *, mode: str | None = None, chunk: int = 4096) -> list:
+    """Adaptively integrate a batch over an affine-parameter interval
+    (integrators.py:127-174), on the device.
+
+    Same semantics as the reference: every sample shares the step sequence
+    (the largest error ratio of the batch controls it), h is clipped to
+    min(h, max_step, lambda_total - elapsed) before each attempt, and the
+    result is the list of accepted states starting with a copy of
+    ``state0``.  Raises RuntimeError("integrate_dense: step size underflow")
+    when the step falls below 1e-14.  ``rhs`` is ``GeodesicRHS(spacetime)``
+    or a ``KerrSpacetime``; its arithmetic build is ``mode`` if given, else
+    the GeodesicRHS's (precise by default).  The kernel appends up to
+    ``chunk`` states per launch and resumes from its saved controller state.
+    """
+    g = _resolve(rhs)
+    s0 = _dev.as_device_f64(state0, (8,), "state0")
+    n = s0.shape[0]
+    chunks = [s0.clone()[None]]
+    if not float(lambda_total) > 0.0:
+        return _as_list(chunks, state0)
+    if n == 0:
+        # the reference's batch maximum of an empty ratio array raises
+        raise ValueError("integrate_dense needs at least one state")
+    lib = _lib.require_device()
+    stream = _lib.stream_handle()
+    dev = s0.device
+    y = torch.empty((8, n), dtype=torch.float64, device=dev)
+    _lib.check(lib.bh_aos_to_soa_f64(_lib.ptr(s0), _lib.ptr(y), n, n, stream), "aos_to_soa")
+    work = torch.empty((24, n), dtype=torch.float64, device=dev)
+    traj = torch.empty((chunk, 8, n), dtype=torch.float64, device=dev)
+    ctl = torch.tensor([0.0, float(initial_step)], dtype=torch.float64, device=dev)
+    count = torch.zeros(1, dtype=torch.int64, device=dev)
+    flag = torch.zeros(1, dtype=torch.int32, device=dev)
+    m = _mode(mode or g.mode)
+    while True:
+        _lib.check(lib.bh_integrate_dense_f64(
+            g.spacetime.mass, g.spacetime.spin, m, float(rtol), float(atol), float(max_step),
+            float(lambda_total), _lib.ptr(y), n, n, _lib.ptr(work), _lib.ptr(traj), chunk,
+            _lib.ptr(ctl), _lib.ptr(count), _lib.ptr(flag), stream), "integrate_dense")
+        k = int(count.item())
+        f = int(flag.item())
+        if k:
+            chunks.append(traj[:k].transpose(1, 2).contiguous())
+        if f == _lib.DENSE_UNDERFLOW:
+            raise RuntimeError("integrate_dense: step size underflow")
+        if f == _lib.DENSE_DONE:
+            break
+    return _as_list(chunks, state0)
+
+
+def _as_list(chunks, ref) -> list:
+    """Concatenated (k, n, 8) chunks as a list of k (n, 8) states, in the
+    caller's array type (one device-to-host copy for NumPy callers)."""
+    allst = _dev.like_input(torch.cat(chunks), ref)
+    return [allst[i] for i in range(allst.shape[0])]
+

@@ -212,6 +212,27 @@ int32_t bh_dp5_step_f64(double mass, double spin, int32_t mode, double rtol,
                         int64_t n, double *new_state, double *error,
                         double *ratio, double *factor, void *stream);
 
+/* integrate_dense (integrators.py:127-174) with the geodesic RHS, one
+ * resumable chunk: all n rays take the same step, set by the largest error
+ * ratio of the batch; h = min(h, max_step, lambda_total - elapsed) before
+ * each attempt, h *= step_factor(max ratio) after it.  In/out state
+ * state_soa [8][ld] (the states at the chunk's start, then at its end) and
+ * ctl [2] = {elapsed affine parameter, step size} (device doubles; start
+ * with {0, initial_step}).  Appends at most cap accepted states to
+ * traj [cap][8][n]; *count (device int64) = states appended; *flag (device
+ * int32) = BH_DENSE_DONE (elapsed >= lambda_total), BH_DENSE_CHUNK_FULL
+ * (call again to continue) or BH_DENSE_UNDERFLOW (h < 1e-14: the
+ * reference raises RuntimeError).  work: [24][ld] device doubles.  One
+ * workgroup (meant for test and analysis batches, as the reference's). */
+#define BH_DENSE_DONE 0
+#define BH_DENSE_CHUNK_FULL 1
+#define BH_DENSE_UNDERFLOW 2
+int32_t bh_integrate_dense_f64(double mass, double spin, int32_t mode, double rtol,
+                               double atol, double max_step, double lambda_total,
+                               double *state_soa, int64_t ld, int64_t n, double *work,
+                               double *traj, int64_t cap, double *ctl, int64_t *count,
+                               int32_t *flag, void *stream);
+
 /* null_momentum_from_velocity (geodesics.py:109-170): positions (n, 3),
  * directions (n, 3) -> covariant momenta (n, 4).  past != 0 picks the
  * past-directed root; normalize != 0 rescales to |p_t| = 1. */

@@ -135,3 +135,92 @@ def test_adjoint_with_affine_end_matches_autograd_oracle(hip):
     scale = np.maximum(np.abs(ref).max(axis=1, keepdims=True), 1e-12)
     assert np.max(np.abs(sg.grad.cpu().numpy() - ref) / scale) < 1e-6
     assert abs(float(ag.grad) - float(go_a)) <= 1e-6 * abs(float(go_a))
+
+
+# ---------------------------------------------------------------------------
+# integrate_dense: the batch-shared step sequence (integrators.py:127-174)
+# ---------------------------------------------------------------------------
+
+def _dense_batch(n=6, seed=11):
+    st = orc.Kerr(spin=0.9)
+    p, v = workloads.near_circular_orbits(n, seed=seed)
+    return orc.build_state(p, orc.timelike_momentum(st, p, v))
+
+
+def test_integrate_dense_single_ray_matches_golden(hip):
+    """n = 1: the batch maximum is the ray's own ratio, so the golden
+    integrate_dense runs pin the whole accepted-step count and final state."""
+    from blackhorizon_amd.integrators import integrate_dense
+
+    st = hip.KerrSpacetime(spin=0.9)
+    for k, (s0, lam, rtol, final, steps) in enumerate(_golden_cases()):
+        out = integrate_dense(st, s0, lam, rtol=rtol)
+        assert len(out) == steps + 1, (k, len(out), steps)
+        np.testing.assert_array_equal(out[0], s0)
+        spread = _oracle_spread(s0, lam, rtol, final)
+        d = float(block_diff(out[-1], final)[0])
+        assert d <= max(1e-7, 10.0 * spread), (k, d, spread)
+
+
+@pytest.mark.parametrize("mode,rtol,tol", [("precise", 1e-10, 1e-12), ("fast", 1e-8, 1e-6)])
+def test_integrate_dense_batch_matches_oracle(hip, mode, rtol, tol):
+    """Six orbits sharing one step sequence: same number of accepted states
+    and every state of the trajectory close to the oracle's.  precise: the
+    reference's rounding except the pow of the step factor, so 1e-12, or 10x
+    the spread an ulp-level change of the step factors (NumPy's pow,
+    random-ulp pow) causes in the oracle's own trajectory.  fast (opt-in
+    here; integrate_dense defaults to precise): FMA rounding of the stages
+    moves the embedded error estimate, whose size is ~ratio * rtol, by
+    ~1e-16 relative to the state, hence the shared step sizes by up to
+    ~1e-16 / (ratio * rtol); measured 1.2e-7 on the coordinate-time block
+    (positions and momenta ~1e-8), bar 1e-6."""
+    from blackhorizon_amd.integrators import GeodesicRHS, integrate_dense
+
+    s0 = _dense_batch()
+    ost = orc.Kerr(spin=0.9)
+
+    def oracle(fn=None):
+        return orc.integrate_dense(lambda y: orc.geodesic_rhs(ost, y), s0, 40.0, rtol=rtol,
+                                   factor_fn=fn)
+
+    ref = oracle()
+    spread = np.zeros(len(ref))
+    for fn in (orc.step_factor, orc.step_factor_ulp_noise(1), orc.step_factor_ulp_noise(2)):
+        alt = oracle(fn)
+        if len(alt) == len(ref):
+            spread = np.maximum(spread, [float(np.max(block_diff(a, b)))
+                                         for a, b in zip(alt, ref)])
+    out = integrate_dense(GeodesicRHS(hip.KerrSpacetime(spin=0.9), mode), s0, 40.0, rtol=rtol)
+    assert len(out) == len(ref)
+    for k, (a, b) in enumerate(zip(out, ref)):
+        assert float(np.max(block_diff(a, b))) <= max(tol, 10.0 * spread[k]), k
+
+
+def test_integrate_dense_chunks_resume_exactly(hip):
+    from blackhorizon_amd.integrators import integrate_dense
+
+    s0 = _dense_batch(n=3, seed=12)
+    st = hip.KerrSpacetime(spin=0.9)
+    whole = integrate_dense(st, s0, 25.0)
+    parts = integrate_dense(st, s0, 25.0, chunk=7)
+    assert len(whole) == len(parts) > 8
+    for a, b in zip(whole, parts):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_integrate_dense_edges(hip):
+    from blackhorizon_amd.integrators import integrate_dense
+
+    st = hip.KerrSpacetime(spin=0.9)
+    s0 = _dense_batch(n=2)
+    out = integrate_dense(st, s0, 0.0)
+    assert len(out) == 1 and np.array_equal(out[0], s0)
+    with pytest.raises(ValueError):
+        integrate_dense(st, s0[:0], 1.0)
+    bad = s0.copy()
+    bad[1, 5] = np.nan  # every ratio is +inf: the step shrinks to underflow, as in the reference
+    with pytest.raises(RuntimeError, match="underflow"):
+        integrate_dense(st, bad, 10.0)
+    # tensors in, tensors out (on the device)
+    out = integrate_dense(st, torch.from_numpy(s0).cuda(), 5.0)
+    assert isinstance(out[-1], torch.Tensor) and out[-1].is_cuda and out[-1].shape == (2, 8)
