@@ -284,6 +284,112 @@ def run_uneven(args) -> dict:
         jobs_per_mode=modes, per_ray_steps_identical=bool(same and steps_b == steps_p))
 
 
+RK4_FLOP_PER_STEP = 4 * 146 + 104  # 4 RHS + stage and update arithmetic, 8 components
+
+
+def rk4_rays(n: int):
+    """Positions and directions of the reference benchmark's rays
+    (examples/benchmark.py:22-31): seed-7 isotropic unit vectors u, photons
+    at 50 u aimed at -u."""
+    rng = np.random.default_rng(7)
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=-1, keepdims=True)
+    return 50.0 * u, -u
+
+
+def rk4_states_device(n: int):
+    """(n, 8) states of the benchmark's rays built on the device (a = 0.9)."""
+    import torch
+
+    from blackhorizon_amd import geodesics, kerr
+
+    pos, dirs = rk4_rays(n)
+    st = kerr.KerrSpacetime(1.0, 0.9)
+    p = torch.from_numpy(pos).cuda()
+    return geodesics.build_state(p, geodesics.null_momentum_from_velocity(
+        st, p, torch.from_numpy(dirs).cuda()))
+
+
+def rk4_states_oracle(n: int):
+    """The same rays built by the oracle (CPU baseline and checker)."""
+    from oracle import bh_oracle as orc
+
+    pos, dirs = rk4_rays(n)
+    st = orc.Kerr(spin=0.9)
+    return orc.build_state(pos, orc.null_momentum_from_velocity(st, pos, dirs))
+
+
+def run_rk4(args) -> dict:
+    """The reference's own throughput benchmark, like for like: fixed-step
+    RK4, h = 0.05, 50 steps of 1e5 rays (examples/benchmark.py:34-62), plus
+    the same at 2^21 rays, which fills the GPU; all 50 steps run in one
+    launch with the state in registers.  The reference publishes 1376-1700
+    ns/ray-step for NumPy FP64 on a CPU and 242.2 ns for CuPy FP64 on an RTX
+    3070 (docs/DESIGN.md:282-285) on this benchmark."""
+    import torch
+
+    from blackhorizon_amd import _lib, integrators, kerr
+
+    lib = _lib.require_device()
+    st = kerr.KerrSpacetime(1.0, 0.9)
+    out = {}
+    cpu = None
+    for n in (100_000, 1 << 21):
+        state = rk4_states_device(n)
+        soa = torch.empty((8, n), dtype=torch.float64, device="cuda")
+        stream = _lib.stream_handle()
+        _lib.check(lib.bh_aos_to_soa_f64(_lib.ptr(state), _lib.ptr(soa), n, n, stream), "soa")
+        h = torch.full((n,), 0.05, dtype=torch.float64, device="cuda")
+        work = soa.clone()
+        cur = torch.cuda.current_stream()
+        for _ in range(args.warmup):
+            work.copy_(soa)
+            _lib.check(lib.bh_rk4_f64(1.0, 0.9, _lib.MODE_FAST, _lib.ptr(work), n, _lib.ptr(h), n,
+                                      50, stream), "rk4")
+        ms = []
+        for _ in range(args.steps):
+            work.copy_(soa)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(cur)
+            _lib.check(lib.bh_rk4_f64(1.0, 0.9, _lib.MODE_FAST, _lib.ptr(work), n, _lib.ptr(h), n,
+                                      50, stream), "rk4")
+            e1.record(cur)
+            e1.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        t = float(np.median(ms))
+        ray_steps = n * 50
+        out[f"rays_{n}"] = dict(ms=t, ns_per_ray_step=t * 1e6 / ray_steps,
+                                ray_steps_per_s=ray_steps / (t * 1e-3),
+                                pct_fp64_vector_peak=100.0 * ray_steps * RK4_FLOP_PER_STEP
+                                / (t * 1e-3) / (FP64_VECTOR_PEAK_TFLOPS * 1e12))
+        print(f"[bench] rk4 {n} rays done", file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        from oracle import bh_oracle as orc
+
+        # the precise build equals the NumPy restatement bit for bit (check)
+        sample = rk4_states_oracle(4096)
+        got = integrators.rk4_steps(integrators.GeodesicRHS(st, "precise"), sample,
+                                    np.full(4096, 0.05), 50)
+        ost = orc.Kerr(spin=0.9)
+        y = rk4_states_oracle(20_000)
+        hh = np.full(y.shape[0], 0.05)
+        t0 = time.perf_counter()
+        for _ in range(50):
+            y = orc.rk4_step(lambda b: orc.geodesic_rhs(ost, b), y, hh)
+        wall = time.perf_counter() - t0
+        cpu = dict(value=20_000 * 50 / wall, unit="ray-steps/s", cores=1, kind="port",
+                   sample="20,000 of the benchmark's rays x 50 RK4 steps, NumPy oracle rk4_step",
+                   ns_per_ray_step=wall / (20_000 * 50) * 1e9)
+        ref = sample.copy()
+        for _ in range(50):
+            ref = orc.rk4_step(lambda b: orc.geodesic_rhs(ost, b), ref, np.full(4096, 0.05))
+        out["precise_equals_oracle_bitwise"] = bool(np.array_equal(got, ref))
+    out["cpu_baseline"] = cpu
+    out["reference_published_ns_per_ray_step"] = {
+        "numpy_fp64_cpu": [1376, 1700], "cupy_fp64_rtx3070": 242.2}
+    return out
+
+
 def run_shadowfit(args) -> dict:
     """512 x 512 differentiable shadow fit at a = 0.7, inclination 45 deg,
     r_obs = 40 M, FOV 30 deg, escape sphere 45 M, rtol 1e-8, checkpoints
@@ -439,7 +545,7 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="hd",
-                    choices=["hd", "hd09", "small", "uneven", "shadowfit", "orbits"])
+                    choices=["hd", "hd09", "small", "uneven", "shadowfit", "orbits", "rk4"])
     ap.add_argument("--orbits", type=int, default=65536, help="orbits: number of orbits")
     ap.add_argument("--proper-time", type=float, default=1.0e4, help="orbits: proper time")
     ap.add_argument("--fit-size", type=int, default=512, help="shadowfit: image side")
@@ -468,6 +574,11 @@ def main(argv=None) -> int:
         res = run_orbits(args)
         print(json.dumps({"metric": METRIC, "workload": "timelike orbits", "mode": args.mode,
                           **res}), flush=True)
+        return 0
+    if args.workload == "rk4":
+        res = run_rk4(args)
+        print(json.dumps({"metric": "ns per ray-step, fixed-step RK4 (reference benchmark)",
+                          "workload": "rk4", **res}), flush=True)
         return 0
     if args.workload == "shadowfit":
         res = run_shadowfit(args)

@@ -87,6 +87,7 @@ SIGNATURES = {
     "bh_geometry_f64": [_D, _D, _I32, _P, _I64, _P, _P, _P, _P, _P, _P],
     "bh_geodesic_rhs_f64": [_D, _D, _I32, _P, _I64, _P, _P],
     "bh_dp5_step_f64": [_D, _D, _I32, _D, _D, _P, _P, _I64, _P, _P, _P, _P, _P],
+    "bh_rk4_f64": [_D, _D, _I32, _P, _I64, _P, _I64, _I64, _P],
     "bh_integrate_dense_f64": [_D, _D, _I32, _D, _D, _D, _D, _P, _I64, _I64, _P, _P, _I64, _P,
                                _P, _P, _P],
     "bh_null_momentum_f64": [_D, _D, _P, _P, _I64, _I32, _I32, _P, _P],

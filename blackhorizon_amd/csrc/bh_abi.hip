@@ -308,6 +308,20 @@ int32_t bh_dp5_step_f64(double mass, double spin, int32_t mode, double rtol, dou
   return hip_status(e, "bh_dp5_step_f64");
 }
 
+int32_t bh_rk4_f64(double mass, double spin, int32_t mode, double *state_soa, int64_t ld,
+                   const double *h, int64_t n, int64_t n_steps, void *stream) {
+  int32_t rc = check_spacetime(mass, spin);
+  if (rc) return rc;
+  if (mode != BH_MODE_FAST && mode != BH_MODE_PRECISE) return fail(BH_EINVAL, "bad mode");
+  if (n < 0 || ld < n || n_steps < 0) return fail(BH_EINVAL, "bad n / ld / n_steps");
+  if (n > 0 && (!state_soa || !h)) return fail(BH_EINVAL, "NULL buffer");
+  JobConst c = spacetime_const(mass, spin);
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = mode == BH_MODE_PRECISE ? precise::launch_rk4(c, state_soa, ld, h, n, n_steps, s)
+                                         : fast::launch_rk4(c, state_soa, ld, h, n, n_steps, s);
+  return hip_status(e, "bh_rk4_f64");
+}
+
 int32_t bh_integrate_dense_f64(double mass, double spin, int32_t mode, double rtol,
                                double atol, double max_step, double lambda_total,
                                double *state_soa, int64_t ld, int64_t n, double *work,

@@ -137,6 +137,8 @@ struct LaunchShape {
   hipError_t launch_rhs_vjp(const JobConst &c, const double *state, const double *v, \
                             int64_t n, double *zbar, double *abar, hipStream_t s); \
   hipError_t launch_integrate_dense(const JobConst &c, const DenseIO &d, hipStream_t s); \
+  hipError_t launch_rk4(const JobConst &c, double *y_soa, int64_t ld, const double *h, \
+                        int64_t n, int64_t n_steps, hipStream_t s);               \
   hipError_t launch_count_outside(const JobConst &c, const double *in, int64_t ld, \
                                   int64_t n, int32_t *count, hipStream_t s);       \
   hipError_t launch_geometry(const JobConst &c, const double *xyz, int64_t n,      \

@@ -83,6 +83,35 @@ def dormand_prince_attempt(rhs, state: Array, h: Array, rtol: float, atol: float
     return tuple(_dev.like_input(t, state) for t in out)
 
 
+def rk4_steps(rhs, state: Array, h: Array, n_steps: int, *, mode: str | None = None) -> Array:
+    """``n_steps`` classical RK4 steps (integrators.py:57-73) per ray in one
+    kernel launch, the state kept in registers between steps; equals
+    applying ``rk4_step`` n_steps times (bitwise in the precise build)."""
+    g = _resolve(rhs)
+    s = _dev.as_device_f64(state, (8,), "state")
+    n = s.shape[0]
+    hv = _dev.as_device_f64(h, None, "h").reshape(-1)
+    if hv.shape[0] != n:
+        raise ValueError("h must have shape (n,)")
+    if n_steps < 0:
+        raise ValueError("n_steps must be >= 0")
+    lib = _lib.require_device()
+    stream = _lib.stream_handle()
+    ld = max(n, 1)
+    soa = torch.empty((8, ld), dtype=torch.float64, device=s.device)
+    _lib.check(lib.bh_aos_to_soa_f64(_lib.ptr(s), _lib.ptr(soa), ld, n, stream), "aos_to_soa")
+    _lib.check(lib.bh_rk4_f64(g.spacetime.mass, g.spacetime.spin, _mode(mode or g.mode),
+                              _lib.ptr(soa), ld, _lib.ptr(hv), n, int(n_steps), stream), "rk4")
+    out = torch.empty_like(s)
+    _lib.check(lib.bh_soa_to_aos_f64(_lib.ptr(soa), ld, _lib.ptr(out), n, stream), "soa_to_aos")
+    return _dev.like_input(out, state)
+
+
+def rk4_step(rhs, state: Array, h: Array) -> Array:
+    """One classical RK4 step, (n, 8) -> (n, 8) (integrators.py:57-73)."""
+    return rk4_steps(rhs, state, h, 1)
+
+
 def integrate_dense(rhs, state0: Array, lambda_total: float, rtol: float = 1e-10,
                     atol: float = 1e-12, initial_step: float = 0.05, max_step: float = 2.0,
                     *, mode: str | None = None, chunk: int = 4096) -> list:

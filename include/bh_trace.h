@@ -212,6 +212,13 @@ int32_t bh_dp5_step_f64(double mass, double spin, int32_t mode, double rtol,
                         int64_t n, double *new_state, double *error,
                         double *ratio, double *factor, void *stream);
 
+/* rk4_step (integrators.py:57-73) with the geodesic RHS, applied n_steps
+ * times per ray (the reference's fixed-step throughput benchmark,
+ * examples/benchmark.py:34-62, loops rk4_step n_steps times): in place on
+ * state_soa [8][ld], per-ray step h (n). */
+int32_t bh_rk4_f64(double mass, double spin, int32_t mode, double *state_soa, int64_t ld,
+                   const double *h, int64_t n, int64_t n_steps, void *stream);
+
 /* integrate_dense (integrators.py:127-174) with the geodesic RHS, one
  * resumable chunk: all n rays take the same step, set by the largest error
  * ratio of the batch; h = min(h, max_step, lambda_total - elapsed) before

@@ -224,3 +224,43 @@ def test_integrate_dense_edges(hip):
     # tensors in, tensors out (on the device)
     out = integrate_dense(st, torch.from_numpy(s0).cuda(), 5.0)
     assert isinstance(out[-1], torch.Tensor) and out[-1].is_cuda and out[-1].shape == (2, 8)
+
+
+# ---------------------------------------------------------------------------
+# rk4_step (integrators.py:57-73), the reference benchmark's stepper
+# ---------------------------------------------------------------------------
+
+def _rk4_inputs():
+    st = orc.Kerr(spin=0.9)
+    rng = np.random.default_rng(7)
+    u = rng.normal(size=(300, 3))
+    u /= np.linalg.norm(u, axis=-1, keepdims=True)
+    photons = orc.build_state(50.0 * u, orc.null_momentum_from_velocity(st, 50.0 * u, -u))
+    orbits = _dense_batch(n=60, seed=13)
+    s = np.concatenate([photons, orbits])
+    h = np.exp(rng.uniform(np.log(1e-3), np.log(0.5), size=s.shape[0]))
+    return st, s, h
+
+
+def test_rk4_step_precise_bitwise_equals_oracle(hip):
+    from blackhorizon_amd.integrators import GeodesicRHS, rk4_step, rk4_steps
+
+    ost, s, h = _rk4_inputs()
+    rhs = GeodesicRHS(hip.KerrSpacetime(spin=0.9), "precise")
+    ref = orc.rk4_step(lambda y: orc.geodesic_rhs(ost, y), s, h)
+    np.testing.assert_array_equal(rk4_step(rhs, s, h), ref)
+    # n fused steps == n applications
+    ref = s
+    for _ in range(20):
+        ref = orc.rk4_step(lambda y: orc.geodesic_rhs(ost, y), ref, h)
+    np.testing.assert_array_equal(rk4_steps(rhs, s, h, 20), ref)
+    np.testing.assert_array_equal(rk4_steps(rhs, s, h, 0), s)
+
+
+def test_rk4_step_fast_close_to_oracle(hip):
+    from blackhorizon_amd.integrators import GeodesicRHS, rk4_step
+
+    ost, s, h = _rk4_inputs()
+    got = rk4_step(GeodesicRHS(hip.KerrSpacetime(spin=0.9), "fast"), s, h)
+    ref = orc.rk4_step(lambda y: orc.geodesic_rhs(ost, y), s, h)
+    assert float(np.max(block_diff(got, ref))) < 1e-13
