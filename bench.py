@@ -390,6 +390,75 @@ def run_rk4(args) -> dict:
     return out
 
 
+def run_offline(args) -> dict:
+    """The offline renderer's hero frame, tracing only: 1920 x 1200 at 2 x 2
+    supersampling = 9.2 M rays, the renderer's defaults (a = 0.9, camera at
+    26 M, inclination 82 deg, FOV 70 deg, rtol 1e-9, retrace at 1e-11 with
+    4 x 12000 steps, disk from the ISCO to 18 M), both passes timed with
+    HIP events (rays generated on the device beforehand).  The reference
+    publishes 1645-1669 s for this frame on an RTX 3070 with CuPy, tracing
+    plus CPU emission (docs/DESIGN.md:466-472, 503-504)."""
+    import torch
+
+    from blackhorizon_amd import _lib, kerr, offline
+
+    _lib.require_device()
+    w, h = (args.offline_size, args.offline_size * 5 // 8)
+    settings = offline.OfflineSettings()
+    cam = offline.FlyCamera.from_orbit(26.0, 82.0)
+    st = kerr.KerrSpacetime(1.0, settings.spin)
+    inner = offline.disk_inner_radius(st)
+    outer = max(settings.disk_outer_radius, inner + 1.0)
+    esc = 1.3 * max(cam.distance_from_origin, outer)
+    s0 = offline.camera_rays(st, cam, w, h, settings)
+    order = torch.from_numpy(offline.pixel_tile_order(w, h, settings.supersample, 4))
+    cur = torch.cuda.current_stream()
+    ms = []
+    for k in range(args.warmup + args.steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(cur)
+        res, retrace = offline.trace_tile_retrace(st, s0, esc, settings, (inner, outer),
+                                                  mode=args.mode, ray_order=order)
+        e1.record(cur)
+        torch.cuda.synchronize()
+        if k >= args.warmup:
+            ms.append(e0.elapsed_time(e1))
+        print(f"[bench] offline pass {k} done", file=sys.stderr, flush=True)
+    t = float(np.median(ms))
+    n = s0.shape[0]
+    att1 = int(res.attempts.sum().item())
+    steps1 = int(res.steps.sum().item())
+    n_re = int(retrace.sum().item())
+    counts = torch.bincount(res.status.long(), minlength=7).tolist()
+    out = dict(rays=n, width=w, height=h, supersample=settings.supersample,
+               mode=args.mode, ms_trace=t, first_pass_attempts=att1, first_pass_ray_steps=steps1,
+               retraced_rays=n_re, status_counts=counts,
+               first_pass_pct_fp64_peak_lower_bound=100.0 * att1 * FLOP_PER_ATTEMPT
+               / (t * 1e-3) / (FP64_VECTOR_PEAK_TFLOPS * 1e12),
+               reference_published_s=[1645, 1669])
+    if not args.no_cpu:
+        from oracle import bh_oracle as orc
+        from oracle import offline_oracle as off
+
+        rng = np.random.default_rng(5)
+        idx = np.sort(rng.choice(n, size=1500, replace=False))
+        ost = orc.Kerr(spin=settings.spin)
+        dirs = off.subpixel_directions(off.FlyCamera.from_orbit(26.0, 82.0), w, h,
+                                       settings.fov_degrees, settings.supersample)[idx]
+        pos = np.tile(cam.position, (len(idx), 1))
+        y0 = orc.build_state(pos, orc.null_momentum_from_velocity(ost, pos, dirs))
+        t0 = time.perf_counter()
+        r, _ = off.trace_tile_retrace(ost, y0, esc, (inner, outer), settings.rtol,
+                                      settings.refine_rtol, settings.max_steps)
+        wall = time.perf_counter() - t0
+        out["cpu_baseline"] = dict(
+            value=int(r.steps.sum()) / wall, unit="first-pass ray-steps/s", cores=1, kind="port",
+            sample="1500 seeded rays of the frame, NumPy oracle of the offline loop, both passes",
+            s_per_ray=wall / len(idx), projected_frame_s=wall / len(idx) * n)
+    return out
+
+
 def run_shadowfit(args) -> dict:
     """512 x 512 differentiable shadow fit at a = 0.7, inclination 45 deg,
     r_obs = 40 M, FOV 30 deg, escape sphere 45 M, rtol 1e-8, checkpoints
@@ -545,10 +614,13 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="hd",
-                    choices=["hd", "hd09", "small", "uneven", "shadowfit", "orbits", "rk4"])
+                    choices=["hd", "hd09", "small", "uneven", "shadowfit", "orbits", "rk4",
+                             "offline"])
     ap.add_argument("--orbits", type=int, default=65536, help="orbits: number of orbits")
     ap.add_argument("--proper-time", type=float, default=1.0e4, help="orbits: proper time")
     ap.add_argument("--fit-size", type=int, default=512, help="shadowfit: image side")
+    ap.add_argument("--offline-size", type=int, default=1920,
+                    help="offline: frame width (height = 5/8 of it)")
     ap.add_argument("--serial-groups", action="store_true",
                     help="uneven: run the fast and precise batches one after the other")
     ap.add_argument("--mode", default="auto", choices=["auto", "fast", "precise"])
@@ -573,6 +645,11 @@ def main(argv=None) -> int:
     if args.workload == "orbits":
         res = run_orbits(args)
         print(json.dumps({"metric": METRIC, "workload": "timelike orbits", "mode": args.mode,
+                          **res}), flush=True)
+        return 0
+    if args.workload == "offline":
+        res = run_offline(args)
+        print(json.dumps({"metric": "offline hero frame tracing time", "workload": "offline",
                           **res}), flush=True)
         return 0
     if args.workload == "rk4":

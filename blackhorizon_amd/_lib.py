@@ -51,6 +51,22 @@ class TraceParams(ctypes.Structure):
     ]
 
 
+class TileParams(ctypes.Structure):
+    """bh_tile_params."""
+
+    _fields_ = [
+        ("disk_inner", ctypes.c_double),
+        ("disk_outer", ctypes.c_double),
+        ("saturate_steps", ctypes.c_int64),
+        ("flags", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+TILE_DISK = 1
+TILE_INTERIOR = 2
+
+
 class TuneParams(ctypes.Structure):
     """bh_tune_params."""
 
@@ -87,6 +103,9 @@ SIGNATURES = {
     "bh_geometry_f64": [_D, _D, _I32, _P, _I64, _P, _P, _P, _P, _P, _P],
     "bh_geodesic_rhs_f64": [_D, _D, _I32, _P, _I64, _P, _P],
     "bh_dp5_step_f64": [_D, _D, _I32, _D, _D, _P, _P, _I64, _P, _P, _P, _P, _P],
+    "bh_trace_tile_f64": [ctypes.POINTER(TraceParams), ctypes.POINTER(TileParams),
+                          ctypes.POINTER(TuneParams), _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P,
+                          _P, _P, _P, _I64, _P],
     "bh_rk4_f64": [_D, _D, _I32, _P, _I64, _P, _I64, _I64, _P],
     "bh_integrate_dense_f64": [_D, _D, _I32, _D, _D, _D, _D, _P, _I64, _I64, _P, _P, _I64, _P,
                                _P, _P, _P],

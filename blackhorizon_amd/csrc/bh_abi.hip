@@ -308,6 +308,41 @@ int32_t bh_dp5_step_f64(double mass, double spin, int32_t mode, double rtol, dou
   return hip_status(e, "bh_dp5_step_f64");
 }
 
+int32_t bh_trace_tile_f64(const bh_trace_params *p, const bh_tile_params *t,
+                          const bh_tune_params *tune, const double *state_in_soa,
+                          int64_t ld_in, double *state_out_soa, int64_t ld_out,
+                          int32_t *status, int64_t *steps, int64_t *attempts,
+                          uint8_t *saturated, double *escape_dir, double *hit_pos,
+                          double *hit_mom, uint64_t *iterations, const int64_t *order,
+                          int64_t n, void *stream) {
+  int32_t rc = check_params(p);
+  if (rc) return rc;
+  if (!t) return fail(BH_EINVAL, "NULL tile parameters");
+  if (n < 0 || ld_in < n || (state_out_soa && ld_out < n)) return fail(BH_EINVAL, "bad n / ld");
+  if (n > 0 && (!state_in_soa || !status || !steps || !attempts || !saturated || !escape_dir ||
+                !hit_pos || !hit_mom))
+    return fail(BH_EINVAL, "NULL buffer");
+  if (t->flags & ~(BH_TILE_DISK | BH_TILE_INTERIOR)) return fail(BH_EINVAL, "bad tile flags");
+  if ((t->flags & BH_TILE_DISK) && !(t->disk_inner <= t->disk_outer))
+    return fail(BH_EINVAL, "disk_inner must not exceed disk_outer");
+  int block = tune && tune->block_size ? tune->block_size : 256;
+  if (block <= 0 || block > 256 || block % 64)
+    return fail(BH_EINVAL, "block_size must be a multiple of 64 in [64, 256]");
+  JobConst c = make_job_const(*p);
+  TileConst tc{t->disk_inner, t->disk_outer, p->capture_radius * 1.002, t->saturate_steps,
+               (t->flags & BH_TILE_DISK) ? 1 : 0, (t->flags & BH_TILE_INTERIOR) ? 1 : 0};
+  TileIO io{state_in_soa, ld_in,   state_out_soa, ld_out,  status,
+            steps,        attempts, saturated,    escape_dir, hit_pos,
+            hit_mom,      (unsigned long long *)iterations, order, n};
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = hipSuccess;
+  if (iterations) e = launch_zero_u64((unsigned long long *)iterations, s);
+  if (e == hipSuccess)
+    e = p->mode == BH_MODE_PRECISE ? precise::launch_trace_tile(c, tc, io, block, s)
+                                   : fast::launch_trace_tile(c, tc, io, block, s);
+  return hip_status(e, "bh_trace_tile_f64");
+}
+
 int32_t bh_rk4_f64(double mass, double spin, int32_t mode, double *state_soa, int64_t ld,
                    const double *h, int64_t n, int64_t n_steps, void *stream) {
   int32_t rc = check_spacetime(mass, spin);

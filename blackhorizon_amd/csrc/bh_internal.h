@@ -95,6 +95,31 @@ struct AdjointIO {
   int32_t K;
 };
 
+// Offline tile tracer (offline/render.py:184-349).
+struct TileConst {
+  double disk_inner, disk_outer;
+  double interior_near;    // capture_radius * 1.002 (render.py:243-246)
+  int64_t saturate_steps;  // int(0.85 * max_steps) (render.py:294-296)
+  int32_t disk, interior;
+};
+
+struct TileIO {
+  const double *in;
+  int64_t ld_in;
+  double *out;  // optional final states
+  int64_t ld_out;
+  int32_t *status;
+  int64_t *steps;
+  int64_t *attempts;
+  uint8_t *saturated;
+  double *escape_dir;  // (n, 3)
+  double *hit_pos;     // (n, 3)
+  double *hit_mom;     // (n, 3)
+  unsigned long long *iterations;
+  const int64_t *order;
+  int64_t n;
+};
+
 // Batch-shared dense integration (integrators.py:127-174), one chunk.
 struct DenseIO {
   double *y;            // [8][ld] in: states at the chunk's start; out: at its end
@@ -137,6 +162,8 @@ struct LaunchShape {
   hipError_t launch_rhs_vjp(const JobConst &c, const double *state, const double *v, \
                             int64_t n, double *zbar, double *abar, hipStream_t s); \
   hipError_t launch_integrate_dense(const JobConst &c, const DenseIO &d, hipStream_t s); \
+  hipError_t launch_trace_tile(const JobConst &c, const TileConst &t, const TileIO &io, \
+                               int block, hipStream_t s);                         \
   hipError_t launch_rk4(const JobConst &c, double *y_soa, int64_t ld, const double *h, \
                         int64_t n, int64_t n_steps, hipStream_t s);               \
   hipError_t launch_count_outside(const JobConst &c, const double *in, int64_t ld, \

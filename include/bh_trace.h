@@ -212,6 +212,44 @@ int32_t bh_dp5_step_f64(double mass, double spin, int32_t mode, double rtol,
                         int64_t n, double *new_state, double *error,
                         double *ratio, double *factor, void *stream);
 
+/* ---- offline tile tracer ---------------------------------------------
+ * The reference's offline renderer traces with its own loop
+ * (offline/render.py:184-309): each pass classifies the current state
+ * first (capture at r <= capture_radius, escape at r >= escape_radius,
+ * MAX_STEPS at steps >= max_steps), then attempts a DP5(4) step
+ * (atol = rtol * 1e-3 is the caller's, initial h = initial_step = 0.1 there;
+ * no min_step test); an accepted step whose z changes sign is refined by 40
+ * bisections along an RK4 sub-step (render.py:312-349) and ends the ray as
+ * BH_DISK when the crossing radius lies in [disk_inner, disk_outer]. */
+#define BH_TILE_DISK 1     /* detect equatorial disk crossings */
+#define BH_TILE_INTERIOR 2 /* interior camera: capture_radius is the stop
+                            * radius, captures end BH_TERMINATED, and rays
+                            * with |p| > 1e3, or |p| > 50 within 1.002 x the
+                            * stop radius, stop too (render.py:236-246) */
+typedef struct bh_tile_params {
+  double disk_inner;
+  double disk_outer;
+  int64_t saturate_steps; /* steps counted as saturated: int(0.85 * max_steps) */
+  int32_t flags;          /* BH_TILE_* */
+  int32_t reserved;
+} bh_tile_params;
+
+/* Trace n rays with the offline loop.  Outputs per ray: status, accepted
+ * steps, attempts, saturated (uint8: MAX_STEPS or steps >= saturate_steps),
+ * escape_dir (n, 3): unit flat-space direction of escaped rays
+ * (render.py:164-181 with p_t from the state), 0 otherwise; hit_pos /
+ * hit_mom (n, 3): crossing position and spatial momentum of disk hits, 0
+ * otherwise; optional final states (SoA, NULL to skip); *iterations (device
+ * uint64) = max attempts.  order as bh_trace_plain_f64.  Replaces
+ * _trace_tile (render.py:184-309) and _refine_crossings (312-349). */
+int32_t bh_trace_tile_f64(const bh_trace_params *p, const bh_tile_params *t,
+                          const bh_tune_params *tune, const double *state_in_soa,
+                          int64_t ld_in, double *state_out_soa, int64_t ld_out,
+                          int32_t *status, int64_t *steps, int64_t *attempts,
+                          uint8_t *saturated, double *escape_dir, double *hit_pos,
+                          double *hit_mom, uint64_t *iterations, const int64_t *order,
+                          int64_t n, void *stream);
+
 /* rk4_step (integrators.py:57-73) with the geodesic RHS, applied n_steps
  * times per ray (the reference's fixed-step throughput benchmark,
  * examples/benchmark.py:34-62, loops rk4_step n_steps times): in place on

@@ -1,0 +1,149 @@
+"""GPU: the offline tile tracer (bh_trace_tile_f64) against the oracle of
+the reference's offline loop, against the Schwarzschild orbit-equation
+quadrature for disk hits, and its retrace and interior-camera modes."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import bh_oracle as orc
+from oracle import offline_oracle as off
+from parity import block_diff
+from test_oracle_offline import _face_on_rays, _quadrature_hit_radius
+
+pytestmark = pytest.mark.gpu
+
+
+def _frame(spin=0.9, w=24, h=16, ss=1, dist=26.0, inc=82.0):
+    st = orc.Kerr(spin=spin)
+    cam = off.FlyCamera.from_orbit(dist, inc)
+    dirs = off.subpixel_directions(cam, w, h, 70.0, ss)
+    pos = np.tile(cam.position, (len(dirs), 1))
+    return st, orc.build_state(pos, orc.null_momentum_from_velocity(st, pos, dirs))
+
+
+def _settings(**kw):
+    from blackhorizon_amd.offline import OfflineSettings
+
+    return OfflineSettings(**kw)
+
+
+def _oracle_spread_status(st, s0, kw):
+    """Rays whose oracle status changes under ulp-level step-factor changes."""
+    ref = off.trace_tile(st, s0, **kw)
+    unstable = np.zeros(len(s0), bool)
+    for fn in (orc.step_factor_libm, orc.step_factor_ulp_noise(1)):
+        alt = off.trace_tile(st, s0, factor_fn=fn, **kw)
+        unstable |= alt.status != ref.status
+    return ref, unstable
+
+
+def _compare(got, ref, unstable, tol):
+    same = ~unstable
+    np.testing.assert_array_equal(got.status[same], ref.status[same])
+    ok = same & (got.status == ref.status)
+    hit = ok & (ref.status == 5)
+    esc = ok & (ref.status == 2)
+    assert hit.sum() > 10 and esc.sum() > 10
+    scale = np.maximum(np.linalg.norm(ref.hit_positions[hit], axis=-1), 1.0)
+    assert np.max(np.linalg.norm(got.hit_positions[hit] - ref.hit_positions[hit], axis=-1)
+                  / scale) <= tol
+    assert np.max(np.abs(got.escape_directions[esc] - ref.escape_directions[esc])) <= tol
+    np.testing.assert_array_equal(got.escape_directions[~esc & ok], 0.0)
+    return ok
+
+
+@pytest.mark.parametrize("mode,tol", [("precise", 1e-9), ("fast", 1e-6)])
+def test_trace_tile_matches_oracle(hip, mode, tol):
+    from blackhorizon_amd import offline
+
+    st, s0 = _frame()
+    inner = st.isco_radius(True)
+    kw = dict(escape_radius=1.3 * 26.0, disk_radii=(inner, 18.0), rtol=1e-9, max_steps=3000)
+    ref, unstable = _oracle_spread_status(st, s0, kw)
+    assert unstable.mean() < 0.02
+    got = offline.trace_tile(hip.KerrSpacetime(spin=0.9), s0, kw["escape_radius"], _settings(),
+                             kw["disk_radii"], kw["rtol"], kw["max_steps"], mode=mode)
+    ok = _compare(got, ref, unstable, tol)
+    if mode == "precise":
+        np.testing.assert_array_equal(got.steps[ok & (ref.status != 5)],
+                                      ref.steps[ok & (ref.status != 5)])
+        np.testing.assert_array_equal(got.saturated[ok], ref.saturated[ok])
+
+
+def test_face_on_disk_hits_match_quadrature(hip):
+    from blackhorizon_amd import offline
+
+    D = 30.0
+    alphas = np.radians(np.linspace(11.0, 18.0, 8))
+    st, s0 = _face_on_rays(D, alphas)
+    got = offline.trace_tile(hip.KerrSpacetime(spin=0.0), s0, 60.0,
+                             _settings(max_step=0.05), (2.5, 40.0), 1e-10, 20000,
+                             mode="precise")
+    assert np.all(got.status == 5)
+    b = D * np.abs(s0[:, 5])
+    for k in range(len(alphas)):
+        r_q = _quadrature_hit_radius(b[k], D)
+        assert abs(np.linalg.norm(got.hit_positions[k]) - r_q) <= 1e-7 * r_q
+        assert abs(got.hit_positions[k, 2]) < 1e-9
+
+
+def test_retrace_matches_oracle(hip):
+    from blackhorizon_amd import offline
+
+    st, s0 = _frame(w=20, h=12)
+    inner = st.isco_radius(True)
+    disk = (inner, 18.0)
+    ref, mask = off.trace_tile_retrace(st, s0, 1.3 * 26.0, disk, rtol=1e-9, refine_rtol=1e-11,
+                                       max_steps=80)
+    assert mask.sum() > 5
+    got, gmask = offline.trace_tile_retrace(
+        hip.KerrSpacetime(spin=0.9), torch.from_numpy(s0).cuda(), 1.3 * 26.0,
+        _settings(rtol=1e-9, refine_rtol=1e-11, max_steps=80), disk, mode="precise")
+    np.testing.assert_array_equal(gmask.cpu().numpy(), mask)
+    np.testing.assert_array_equal(got.status.cpu().numpy(), ref.status)
+    np.testing.assert_array_equal(got.saturated.cpu().numpy(), ref.saturated)
+    hit = ref.status == 5
+    assert float(np.max(np.abs(got.hit_positions.cpu().numpy()[hit] - ref.hit_positions[hit]))) \
+        <= 1e-8
+
+
+def test_interior_mode_matches_oracle(hip):
+    """Rays starting inside the Schwarzschild horizon (r = 1.5) stop at the
+    singularity radius 0.05 or when their momentum diverges (TERMINATED)."""
+    from blackhorizon_amd import offline
+
+    st = orc.Kerr(spin=0.0)
+    rng = np.random.default_rng(4)
+    u = rng.normal(size=(200, 3))
+    u /= np.linalg.norm(u, axis=-1, keepdims=True)
+    d = rng.normal(size=(200, 3))
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    pos = 1.5 * u
+    # unnormalised covectors: inside the horizon p_t may vanish
+    s0 = orc.build_state(pos, orc.null_momentum_from_velocity(st, pos, d, normalize=False))
+    assert np.all(np.isfinite(s0))
+    kw = dict(escape_radius=100.0, disk_radii=None, rtol=1e-9, max_steps=2000,
+              interior_stop=0.05)
+    ref = off.trace_tile(st, s0, **kw)
+    assert np.all(ref.status == 6)
+    got = offline.trace_tile(hip.KerrSpacetime(spin=0.0), s0, 100.0, _settings(), None, 1e-9,
+                             2000, interior_stop=0.05, mode="precise")
+    np.testing.assert_array_equal(got.status, ref.status)
+    np.testing.assert_array_equal(got.steps, ref.steps)
+
+
+def test_ray_order_and_launch_shape_do_not_change_results(hip):
+    from blackhorizon_amd import offline
+    from blackhorizon_amd.tracer import Tune
+
+    st, s0 = _frame(w=16, h=12, ss=2)
+    args = (hip.KerrSpacetime(spin=0.9), s0, 1.3 * 26.0, _settings(), (st.isco_radius(True), 18.0),
+            1e-8, 3000)
+    a = offline.trace_tile(*args)
+    order = offline.pixel_tile_order(16, 12, 2, 4)
+    assert np.array_equal(np.sort(order), np.arange(len(s0)))
+    b = offline.trace_tile(*args, ray_order=order, tune=Tune(128, 1, 1))
+    for name in ("status", "escape_directions", "hit_positions", "hit_momenta", "saturated",
+                 "steps", "attempts"):
+        np.testing.assert_array_equal(getattr(a, name), getattr(b, name), err_msg=name)
