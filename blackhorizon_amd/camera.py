@@ -81,6 +81,36 @@ class PinholeCamera:
         o = numpy.asarray(self.position, dtype=numpy.float64)
         return numpy.broadcast_to(o, (self.height * self.width, 3)).copy()
 
+    def device_states(self, spacetime, time_orientation: str = "past"):
+        """(height * width, 8) photon states of every pixel built on the
+        device in one kernel -- the directions of ray_directions(), unit
+        |p_t| covectors (null_momentum_from_velocity) and build_state at
+        t = 0 -- without host arrays or a host-to-device copy.  Equal to
+        those three steps done separately (bitwise)."""
+        import torch
+
+        from . import _dev, _lib
+
+        if time_orientation not in ("past", "future"):
+            raise ValueError("time_orientation must be 'past' or 'future'")
+        lib = _lib.require_device()
+        n = self.width * self.height
+        dev = _dev.device()
+        soa = torch.empty((8, n), dtype=torch.float64, device=dev)
+        half_w = math.tan(math.radians(self.fov_degrees) / 2.0)
+        half_h = half_w * self.height / self.width
+        cam = numpy.concatenate([numpy.asarray(self.position, dtype=numpy.float64),
+                                 self.forward, self.right, self.up]).astype(numpy.float64)
+        stream = _lib.stream_handle()
+        _lib.check(lib.bh_camera_rays_f64(spacetime.mass, spacetime.spin,
+                                          cam.ctypes.data, half_w, half_h, self.width,
+                                          self.height, 1 if time_orientation == "past" else 0,
+                                          _lib.ptr(soa), n, stream), "camera_rays")
+        out = torch.empty((n, 8), dtype=torch.float64, device=dev)
+        _lib.check(lib.bh_soa_to_aos_f64(_lib.ptr(soa), n, _lib.ptr(out), n, stream),
+                   "soa_to_aos")
+        return out
+
     def tile_order(self, tile: int = 8) -> numpy.ndarray:
         """Permutation of ray indices that lists the pixels tile by tile
         (tile x tile blocks, row-major inside and across tiles; partial

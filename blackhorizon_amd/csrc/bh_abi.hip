@@ -380,6 +380,44 @@ int32_t bh_integrate_dense_f64(double mass, double spin, int32_t mode, double rt
   return hip_status(e, "bh_integrate_dense_f64");
 }
 
+int32_t bh_camera_rays_f64(double mass, double spin, const double *camera, double half_w,
+                           double half_h, int64_t width, int64_t height, int32_t past,
+                           double *state_soa, int64_t ld, void *stream) {
+  int32_t rc = check_spacetime(mass, spin);
+  if (rc) return rc;
+  if (!camera) return fail(BH_EINVAL, "camera is NULL");
+  if (width < 0 || height < 0 || (height > 0 && width > ((int64_t)1 << 40) / height))
+    return fail(BH_EINVAL, "bad width / height");
+  if (ld < width * height) return fail(BH_EINVAL, "ld < width * height");
+  if (width * height > 0 && !state_soa) return fail(BH_EINVAL, "NULL buffer");
+  CameraConst cam;
+  for (int j = 0; j < 3; ++j) {
+    cam.pos[j] = camera[j];
+    cam.fwd[j] = camera[3 + j];
+    cam.right[j] = camera[6 + j];
+    cam.up[j] = camera[9 + j];
+  }
+  cam.half_w = half_w;
+  cam.half_h = half_h;
+  JobConst c = spacetime_const(mass, spin);
+  return hip_status(precise::launch_camera_rays(c, cam, width, height, past ? 1 : 0, state_soa,
+                                                ld, (hipStream_t)stream),
+                    "bh_camera_rays_f64");
+}
+
+int32_t bh_shade_checker_u8(double mass, double spin, const double *state_soa, int64_t ld,
+                            const int32_t *status, int64_t n, int32_t cells, uint8_t *rgb,
+                            void *stream) {
+  int32_t rc = check_spacetime(mass, spin);
+  if (rc) return rc;
+  if (n < 0 || ld < n || cells < 1) return fail(BH_EINVAL, "bad n / ld / cells");
+  if (n > 0 && (!state_soa || !status || !rgb)) return fail(BH_EINVAL, "NULL buffer");
+  JobConst c = spacetime_const(mass, spin);
+  return hip_status(precise::launch_shade_checker(c, state_soa, ld, status, n, cells, rgb,
+                                                  (hipStream_t)stream),
+                    "bh_shade_checker_u8");
+}
+
 int32_t bh_null_momentum_f64(double mass, double spin, const double *positions,
                              const double *directions, int64_t n, int32_t past,
                              int32_t normalize, double *momenta, void *stream) {

@@ -95,6 +95,12 @@ struct AdjointIO {
   int32_t K;
 };
 
+// Pinhole camera: position, orthonormal basis and half extents.
+struct CameraConst {
+  double pos[3], fwd[3], right[3], up[3];
+  double half_w, half_h;
+};
+
 // Offline tile tracer (offline/render.py:184-349).
 struct TileConst {
   double disk_inner, disk_outer;
@@ -164,6 +170,12 @@ struct LaunchShape {
   hipError_t launch_integrate_dense(const JobConst &c, const DenseIO &d, hipStream_t s); \
   hipError_t launch_trace_tile(const JobConst &c, const TileConst &t, const TileIO &io, \
                                int block, hipStream_t s);                         \
+  hipError_t launch_camera_rays(const JobConst &c, const CameraConst &cam, int64_t width, \
+                                int64_t height, int past, double *soa, int64_t ld, \
+                                hipStream_t s);                                    \
+  hipError_t launch_shade_checker(const JobConst &c, const double *soa, int64_t ld, \
+                                  const int32_t *status, int64_t n, int cells,     \
+                                  uint8_t *rgb, hipStream_t s);                     \
   hipError_t launch_rk4(const JobConst &c, double *y_soa, int64_t ld, const double *h, \
                         int64_t n, int64_t n_steps, hipStream_t s);               \
   hipError_t launch_count_outside(const JobConst &c, const double *in, int64_t ld, \

@@ -278,6 +278,25 @@ int32_t bh_integrate_dense_f64(double mass, double spin, int32_t mode, double rt
                                double *traj, int64_t cap, double *ctl, int64_t *count,
                                int32_t *flag, void *stream);
 
+/* PinholeCamera rays built on the device (camera.py:96-120, then
+ * null_momentum_from_velocity and build_state, geodesics.py:25-39 and
+ * 109-170): camera = {position[3], forward[3], right[3], up[3]} (host
+ * doubles), half_w = tan(fov / 2), half_h = half_w * height / width; pixel
+ * i = row * width + col gets state (0, position, p) in state_soa [8][ld];
+ * past != 0 for past-directed photons. */
+int32_t bh_camera_rays_f64(double mass, double spin, const double *camera, double half_w,
+                           double half_h, int64_t width, int64_t height, int32_t past,
+                           double *state_soa, int64_t ld, void *stream);
+
+/* render_image (imaging.py:53-91) on the device: RGB uint8 (n, 3) per ray
+ * from final states (SoA) and status -- captured (0,0,0), escaped by the
+ * celestial checkerboard of the final coordinate-velocity direction with
+ * `cells` polar cells (imaging.py:27-50), MAX_STEPS (120,32,32), any other
+ * status (200,0,200). */
+int32_t bh_shade_checker_u8(double mass, double spin, const double *state_soa, int64_t ld,
+                            const int32_t *status, int64_t n, int32_t cells, uint8_t *rgb,
+                            void *stream);
+
 /* null_momentum_from_velocity (geodesics.py:109-170): positions (n, 3),
  * directions (n, 3) -> covariant momenta (n, 4).  past != 0 picks the
  * past-directed root; normalize != 0 rescales to |p_t| = 1. */

@@ -587,3 +587,19 @@ def checker_parity(st: Kerr, states):
     i_theta = np.floor(theta / np.pi * cells).astype(np.int64)
     i_phi = np.floor((phi + np.pi) / (2.0 * np.pi) * 2 * cells).astype(np.int64)
     return ((i_theta + i_phi) % 2).astype(bool)
+
+
+def render_image(st: Kerr, states, status, width, height):
+    """imaging.py:53-91: RGB uint8 (height, width, 3) of traced rays."""
+    colors = {"shadow": (0, 0, 0), "dark": (24, 30, 52), "light": (104, 128, 178),
+              "max_steps": (120, 32, 32), "failed": (200, 0, 200)}
+    n = width * height
+    pixels = np.empty((n, 3), dtype=np.uint8)
+    pixels[:] = colors["failed"]
+    esc = status == int(RayStatus.ESCAPED)
+    pixels[status == int(RayStatus.CAPTURED)] = colors["shadow"]
+    light = checker_parity(st, states[esc])
+    pixels[esc] = np.where(light[:, None], colors["light"], colors["dark"]).astype(np.uint8)
+    pixels[status == int(RayStatus.MAX_STEPS)] = colors["max_steps"]
+    return pixels.reshape(height, width, 3)
+

@@ -71,6 +71,7 @@ def test_trace_tile_matches_oracle(hip, mode, tol):
         np.testing.assert_array_equal(got.saturated[ok], ref.saturated[ok])
 
 
+@pytest.mark.filterwarnings("ignore::scipy.integrate.IntegrationWarning")
 def test_face_on_disk_hits_match_quadrature(hip):
     from blackhorizon_amd import offline
 
