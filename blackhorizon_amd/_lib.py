@@ -108,6 +108,7 @@ SIGNATURES = {
                           _P, _P, _P, _I64, _P],
     "bh_camera_rays_f64": [_D, _D, _P, _D, _D, _I64, _I64, _I32, _P, _I64, _P],
     "bh_shade_checker_u8": [_D, _D, _P, _I64, _P, _I64, _I32, _P, _P],
+    "bh_midpoint_f64": [_D, _D, _I32, _P, _I64, _P, _I64, _I64, _I32, _P],
     "bh_rk4_f64": [_D, _D, _I32, _P, _I64, _P, _I64, _I64, _P],
     "bh_integrate_dense_f64": [_D, _D, _I32, _D, _D, _D, _D, _P, _I64, _I64, _P, _P, _I64, _P,
                                _P, _P, _P],

@@ -308,6 +308,24 @@ int32_t bh_dp5_step_f64(double mass, double spin, int32_t mode, double rtol, dou
   return hip_status(e, "bh_dp5_step_f64");
 }
 
+int32_t bh_midpoint_f64(double mass, double spin, int32_t mode, double *state_soa, int64_t ld,
+                        const double *h, int64_t n, int64_t n_steps, int32_t iterations,
+                        void *stream) {
+  int32_t rc = check_spacetime(mass, spin);
+  if (rc) return rc;
+  if (mode != BH_MODE_FAST && mode != BH_MODE_PRECISE) return fail(BH_EINVAL, "bad mode");
+  if (n < 0 || ld < n || n_steps < 0 || iterations < 0)
+    return fail(BH_EINVAL, "bad n / ld / n_steps / iterations");
+  if (n > 0 && (!state_soa || !h)) return fail(BH_EINVAL, "NULL buffer");
+  JobConst c = spacetime_const(mass, spin);
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e =
+      mode == BH_MODE_PRECISE
+          ? precise::launch_midpoint(c, state_soa, ld, h, n, n_steps, iterations, s)
+          : fast::launch_midpoint(c, state_soa, ld, h, n, n_steps, iterations, s);
+  return hip_status(e, "bh_midpoint_f64");
+}
+
 int32_t bh_trace_tile_f64(const bh_trace_params *p, const bh_tile_params *t,
                           const bh_tune_params *tune, const double *state_in_soa,
                           int64_t ld_in, double *state_out_soa, int64_t ld_out,

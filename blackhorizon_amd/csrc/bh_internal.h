@@ -170,6 +170,8 @@ struct LaunchShape {
   hipError_t launch_integrate_dense(const JobConst &c, const DenseIO &d, hipStream_t s); \
   hipError_t launch_trace_tile(const JobConst &c, const TileConst &t, const TileIO &io, \
                                int block, hipStream_t s);                         \
+  hipError_t launch_midpoint(const JobConst &c, double *y_soa, int64_t ld, const double *h, \
+                             int64_t n, int64_t n_steps, int iterations, hipStream_t s); \
   hipError_t launch_camera_rays(const JobConst &c, const CameraConst &cam, int64_t width, \
                                 int64_t height, int past, double *soa, int64_t ld, \
                                 hipStream_t s);                                    \

@@ -83,10 +83,7 @@ def dormand_prince_attempt(rhs, state: Array, h: Array, rtol: float, atol: float
     return tuple(_dev.like_input(t, state) for t in out)
 
 
-def rk4_steps(rhs, state: Array, h: Array, n_steps: int, *, mode: str | None = None) -> Array:
-    """``n_steps`` classical RK4 steps (integrators.py:57-73) per ray in one
-    kernel launch, the state kept in registers between steps; equals
-    applying ``rk4_step`` n_steps times (bitwise in the precise build)."""
+def _fixed_steps(rhs, state: Array, h: Array, n_steps: int, mode, launch, what: str) -> Array:
     g = _resolve(rhs)
     s = _dev.as_device_f64(state, (8,), "state")
     n = s.shape[0]
@@ -100,11 +97,43 @@ def rk4_steps(rhs, state: Array, h: Array, n_steps: int, *, mode: str | None = N
     ld = max(n, 1)
     soa = torch.empty((8, ld), dtype=torch.float64, device=s.device)
     _lib.check(lib.bh_aos_to_soa_f64(_lib.ptr(s), _lib.ptr(soa), ld, n, stream), "aos_to_soa")
-    _lib.check(lib.bh_rk4_f64(g.spacetime.mass, g.spacetime.spin, _mode(mode or g.mode),
-                              _lib.ptr(soa), ld, _lib.ptr(hv), n, int(n_steps), stream), "rk4")
+    _lib.check(launch(lib, g.spacetime, _mode(mode or g.mode), soa, ld, hv, n, int(n_steps),
+                      stream), what)
     out = torch.empty_like(s)
     _lib.check(lib.bh_soa_to_aos_f64(_lib.ptr(soa), ld, _lib.ptr(out), n, stream), "soa_to_aos")
     return _dev.like_input(out, state)
+
+
+def rk4_steps(rhs, state: Array, h: Array, n_steps: int, *, mode: str | None = None) -> Array:
+    """``n_steps`` classical RK4 steps (integrators.py:57-73) per ray in one
+    kernel launch, the state kept in registers between steps; equals
+    applying ``rk4_step`` n_steps times (bitwise in the precise build)."""
+    def launch(lib, st, m, soa, ld, hv, n, k, stream):
+        return lib.bh_rk4_f64(st.mass, st.spin, m, _lib.ptr(soa), ld, _lib.ptr(hv), n, k, stream)
+
+    return _fixed_steps(rhs, state, h, n_steps, mode, launch, "rk4")
+
+
+def implicit_midpoint_steps(rhs, state: Array, h: Array, n_steps: int, iterations: int = 4, *,
+                            mode: str | None = None) -> Array:
+    """``n_steps`` implicit midpoint steps (integrators.py:177-207) per ray
+    in one launch; equals applying ``implicit_midpoint_step`` n_steps times
+    (bitwise in the precise build)."""
+    if iterations < 0:
+        raise ValueError("iterations must be >= 0")
+
+    def launch(lib, st, m, soa, ld, hv, n, k, stream):
+        return lib.bh_midpoint_f64(st.mass, st.spin, m, _lib.ptr(soa), ld, _lib.ptr(hv), n, k,
+                                   int(iterations), stream)
+
+    return _fixed_steps(rhs, state, h, n_steps, mode, launch, "implicit_midpoint")
+
+
+def implicit_midpoint_step(rhs, state: Array, h: Array, iterations: int = 4) -> Array:
+    """One implicit midpoint step, symplectic for Hamiltonian flows: solves
+    y1 = y0 + h f((y0 + y1) / 2) by ``iterations`` fixed-point sweeps from an
+    explicit half step (integrators.py:177-207)."""
+    return implicit_midpoint_steps(rhs, state, h, 1, iterations)
 
 
 def rk4_step(rhs, state: Array, h: Array) -> Array:

@@ -257,6 +257,14 @@ int32_t bh_trace_tile_f64(const bh_trace_params *p, const bh_tile_params *t,
 int32_t bh_rk4_f64(double mass, double spin, int32_t mode, double *state_soa, int64_t ld,
                    const double *h, int64_t n, int64_t n_steps, void *stream);
 
+/* implicit_midpoint_step (integrators.py:177-207) with the geodesic RHS,
+ * `iterations` fixed-point sweeps, applied n_steps times per ray, in place
+ * on state_soa [8][ld]; per-ray step h (n).  Symplectic: bounded energy
+ * error over long orbits. */
+int32_t bh_midpoint_f64(double mass, double spin, int32_t mode, double *state_soa, int64_t ld,
+                        const double *h, int64_t n, int64_t n_steps, int32_t iterations,
+                        void *stream);
+
 /* integrate_dense (integrators.py:127-174) with the geodesic RHS, one
  * resumable chunk: all n rays take the same step, set by the largest error
  * ratio of the batch; h = min(h, max_step, lambda_total - elapsed) before

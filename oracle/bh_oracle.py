@@ -318,6 +318,15 @@ def rk4_step(rhs, state, h):
     return state + (hh / 6.0) * (k1 + 2.0 * k2 + 2.0 * k3 + k4)
 
 
+def implicit_midpoint_step(rhs, state, h, iterations=4):
+    """Implicit midpoint by fixed-point iteration (integrators.py:177-207)."""
+    hh = h[:, None]
+    midpoint = state + 0.5 * hh * rhs(state)
+    for _ in range(iterations):
+        midpoint = state + 0.5 * hh * rhs(midpoint)
+    return state + hh * rhs(midpoint)
+
+
 def dormand_prince_step(rhs, state, h):
     """Embedded DP5(4) step -> (fifth-order state, error); integrators.py:76-107."""
     hh = h[:, None]

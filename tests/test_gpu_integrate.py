@@ -264,3 +264,100 @@ def test_rk4_step_fast_close_to_oracle(hip):
     got = rk4_step(GeodesicRHS(hip.KerrSpacetime(spin=0.9), "fast"), s, h)
     ref = orc.rk4_step(lambda y: orc.geodesic_rhs(ost, y), s, h)
     assert float(np.max(block_diff(got, ref))) < 1e-13
+
+
+# ---------------------------------------------------------------------------
+# implicit midpoint (integrators.py:177-207) and tidal-disruption debris
+# ---------------------------------------------------------------------------
+
+def _orbit_state_14():
+    """test_offline.py:37-44: a = 0.9, r = 14 M, dx/dt = (0, 0.21, 0.02)."""
+    st = orc.Kerr(spin=0.9)
+    pos = np.array([[14.0, 0.0, 0.0]])
+    return st, orc.build_state(pos, orc.timelike_momentum(st, pos, np.array([[0.0, 0.21, 0.02]])))
+
+
+def test_implicit_midpoint_bitwise_equals_oracle(hip):
+    from blackhorizon_amd.integrators import GeodesicRHS, implicit_midpoint_step, \
+        implicit_midpoint_steps
+
+    ost, s, h = _rk4_inputs()
+    rhs = GeodesicRHS(hip.KerrSpacetime(spin=0.9), "precise")
+    f = lambda y: orc.geodesic_rhs(ost, y)  # noqa: E731
+    np.testing.assert_array_equal(implicit_midpoint_step(rhs, s, h),
+                                  orc.implicit_midpoint_step(f, s, h))
+    ref = s
+    for _ in range(10):
+        ref = orc.implicit_midpoint_step(f, ref, h, iterations=6)
+    np.testing.assert_array_equal(implicit_midpoint_steps(rhs, s, h, 10, iterations=6), ref)
+
+
+def test_implicit_midpoint_second_order(hip):
+    """test_offline.py:46-67: halving the step cuts the one-orbit position
+    error about 4x."""
+    from blackhorizon_amd.integrators import implicit_midpoint_steps
+
+    _, s0 = _orbit_state_14()
+    st = hip.KerrSpacetime(spin=0.9)
+
+    def endpoint(h, steps):
+        return implicit_midpoint_steps(st, s0, np.array([h]), steps, iterations=6)
+
+    ref = endpoint(0.025, 4000)
+    e_coarse = np.abs(endpoint(0.2, 500)[0, 1:4] - ref[0, 1:4]).max()
+    e_fine = np.abs(endpoint(0.1, 1000)[0, 1:4] - ref[0, 1:4]).max()
+    assert 1.6 < np.log2(e_coarse / e_fine) < 2.6
+
+
+def test_midpoint_energy_bounded_where_rk4_drifts(hip):
+    """test_offline.py:69-107: 30,000 steps of h = 0.6; the RK4 Hamiltonian
+    error grows secularly, the midpoint one oscillates and stays bounded."""
+    from blackhorizon_amd.integrators import implicit_midpoint_steps, rk4_steps
+
+    ost, s0 = _orbit_state_14()
+    st = hip.KerrSpacetime(spin=0.9)
+    h = np.array([0.6])
+    samples = {}
+    for name, stepper in (("rk4", rk4_steps), ("midpoint", implicit_midpoint_steps)):
+        state, errors = s0, []
+        for _ in range(12):
+            state = stepper(st, state, h, 2500)
+            errors.append(abs(float(orc.hamiltonian(ost, state)[0]) + 0.5))
+        samples[name] = np.asarray(errors)
+    rk4 = samples["rk4"]
+    assert np.all(np.diff(rk4) > 0.0) and rk4[-1] > 5.0 * rk4[0]
+    mid = samples["midpoint"]
+    assert float(mid[-1]) < 0.05 * float(mid.max()) and float(mid.max()) < 1e-2
+
+
+def test_debris_stream_energy_spread(hip):
+    """test_dynamics.py:211-221."""
+    from blackhorizon_amd import dynamics
+
+    st = hip.KerrSpacetime(spin=0.9)
+    stream = dynamics.generate_debris_stream(st, star_mass=1e-6, star_radius=0.5,
+                                             n_particles=4000, seed=3)
+    assert 0.3 < stream.bound_fraction < 0.7
+    spread = float(np.std(stream.specific_energies))
+    analytic = dynamics.energy_spread_estimate(1.0, 0.5, stream.tidal_radius)
+    assert 0.1 * analytic < spread < 3.0 * analytic
+    # the centre moves at the marginally bound speed: E = 1 there
+    v = dynamics.marginally_bound_speed(st, [stream.tidal_radius, 0.0, 0.0], [0.0, 1.0, 0.0])
+    ost = orc.Kerr(spin=0.9)
+    p = orc.timelike_momentum(ost, np.array([[stream.tidal_radius, 0.0, 0.0]]),
+                              np.array([[0.0, v, 0.0]]))
+    assert abs(-p[0, 0] - 1.0) < 1e-12
+
+
+def test_debris_evolves_on_valid_geodesics(hip):
+    """test_dynamics.py:223-236, 400 RK4 steps of h = 0.5 in one launch."""
+    from blackhorizon_amd import dynamics
+    from blackhorizon_amd.integrators import rk4_steps
+
+    st = hip.KerrSpacetime(spin=0.9)
+    stream = dynamics.generate_debris_stream(st, star_mass=1e-6, star_radius=0.5,
+                                             n_particles=64, seed=5, device_states=True)
+    out = rk4_steps(st, stream.states, torch.full((64,), 0.5, dtype=torch.float64,
+                                                  device="cuda"), 400)
+    np.testing.assert_allclose(orc.hamiltonian(orc.Kerr(spin=0.9), out.cpu().numpy()), -0.5,
+                               atol=1e-5)

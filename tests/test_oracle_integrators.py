@@ -30,6 +30,16 @@ def test_orders():
     assert 4.6 < np.log2(e[0] / e[1]) < 5.4 and 4.6 < np.log2(e[1] / e[2]) < 5.4
     e = [_run(orc.rk4_step, n) for n in (100, 200, 400)]
     assert 3.7 < np.log2(e[0] / e[1]) < 4.3 and 3.7 < np.log2(e[1] / e[2]) < 4.3
+    # implicit midpoint: second order, and with enough sweeps it is the exact
+    # midpoint rule, which for a linear oscillator is the Cayley map
+    # (norm-preserving: |y| stays 1 to rounding)
+    mid = lambda f, s, h: orc.implicit_midpoint_step(f, s, h, iterations=30)  # noqa: E731
+    e = [_run(mid, n) for n in (100, 200, 400)]
+    assert 1.9 < np.log2(e[0] / e[1]) < 2.1 and 1.9 < np.log2(e[1] / e[2]) < 2.1
+    s = np.array([[1.0, 0.0]])
+    for _ in range(1000):
+        s = mid(harmonic, s, np.full(1, 0.3))
+    assert abs(np.hypot(s[0, 0], s[0, 1]) - 1.0) < 1e-12
 
 
 def test_tableau_order_conditions():
