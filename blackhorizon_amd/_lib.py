@@ -63,6 +63,15 @@ class TileParams(ctypes.Structure):
     ]
 
 
+class ShaderParams(ctypes.Structure):
+    """bh_shader_params."""
+
+    _fields_ = [(name, ctypes.c_double) for name in (
+        "mass", "spin", "capture_radius", "escape_radius", "step_scale", "min_step", "max_step",
+        "momentum_bailout", "disk_inner", "disk_outer")] + [
+        ("max_steps", ctypes.c_int64), ("flags", ctypes.c_int32), ("mode", ctypes.c_int32)]
+
+
 TILE_DISK = 1
 TILE_INTERIOR = 2
 
@@ -108,6 +117,8 @@ SIGNATURES = {
                           _P, _P, _P, _I64, _P],
     "bh_camera_rays_f64": [_D, _D, _P, _D, _D, _I64, _I64, _I32, _P, _I64, _P],
     "bh_shade_checker_u8": [_D, _D, _P, _I64, _P, _I64, _I32, _P, _P],
+    "bh_trace_shader_f64": [ctypes.POINTER(ShaderParams), _P, _I64, _P, _I64, _P, _P, _P, _I64,
+                            _P],
     "bh_midpoint_f64": [_D, _D, _I32, _P, _I64, _P, _I64, _I64, _I32, _P],
     "bh_rk4_f64": [_D, _D, _I32, _P, _I64, _P, _I64, _I64, _P],
     "bh_integrate_dense_f64": [_D, _D, _I32, _D, _D, _D, _D, _P, _I64, _I64, _P, _P, _I64, _P,

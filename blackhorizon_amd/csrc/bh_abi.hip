@@ -308,6 +308,37 @@ int32_t bh_dp5_step_f64(double mass, double spin, int32_t mode, double rtol, dou
   return hip_status(e, "bh_dp5_step_f64");
 }
 
+int32_t bh_trace_shader_f64(const bh_shader_params *p, const double *state_in_soa, int64_t ld_in,
+                            double *state_out_soa, int64_t ld_out, int32_t *status, double *hit,
+                            const int64_t *order, int64_t n, void *stream) {
+  if (!p) return fail(BH_EINVAL, "params is NULL");
+  int32_t rc = check_spacetime(p->mass, p->spin);
+  if (rc) return rc;
+  if (p->mode != BH_MODE_FAST && p->mode != BH_MODE_PRECISE) return fail(BH_EINVAL, "bad mode");
+  if (p->max_steps < 0 || p->max_steps > ((int64_t)1 << 40)) return fail(BH_EINVAL, "bad max_steps");
+  if (p->flags & ~(BH_TILE_DISK | BH_TILE_INTERIOR)) return fail(BH_EINVAL, "bad flags");
+  if (n < 0 || ld_in < n || (state_out_soa && ld_out < n)) return fail(BH_EINVAL, "bad n / ld");
+  if (n > 0 && (!state_in_soa || !status)) return fail(BH_EINVAL, "NULL buffer");
+  bh_trace_params tp;
+  memset(&tp, 0, sizeof(tp));
+  tp.mass = p->mass;
+  tp.spin = p->spin;
+  tp.capture_radius = p->capture_radius;
+  tp.escape_radius = p->escape_radius;
+  JobConst c = make_job_const(tp);
+  const double r_plus = p->mass + std::sqrt(p->mass * p->mass - p->spin * p->spin);
+  ShaderConst sp{r_plus,        p->step_scale, p->min_step, p->max_step,
+                 p->momentum_bailout * p->momentum_bailout,
+                 p->disk_inner, p->disk_outer, p->capture_radius * 1.002,
+                 p->max_steps,  (p->flags & BH_TILE_DISK) ? 1 : 0,
+                 (p->flags & BH_TILE_INTERIOR) ? 1 : 0};
+  ShaderIO io{state_in_soa, ld_in, state_out_soa, ld_out, status, hit, order, n};
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = p->mode == BH_MODE_PRECISE ? precise::launch_trace_shader(c, sp, io, s)
+                                            : fast::launch_trace_shader(c, sp, io, s);
+  return hip_status(e, "bh_trace_shader_f64");
+}
+
 int32_t bh_midpoint_f64(double mass, double spin, int32_t mode, double *state_soa, int64_t ld,
                         const double *h, int64_t n, int64_t n_steps, int32_t iterations,
                         void *stream) {

@@ -126,6 +126,25 @@ struct TileIO {
   int64_t n;
 };
 
+// The real-time shader's fixed-step algorithm (realtime/reference.py:41-188).
+struct ShaderConst {
+  double r_plus, step_scale, min_step, max_step, bailout2;
+  double disk_inner, disk_outer, interior_near;
+  int64_t max_steps;
+  int32_t disk, interior;
+};
+
+struct ShaderIO {
+  const double *in;
+  int64_t ld_in;
+  double *out;  // optional final states
+  int64_t ld_out;
+  int32_t *status;
+  double *hit;  // optional (n, 8): crossing radius, position (3), momentum (4)
+  const int64_t *order;
+  int64_t n;
+};
+
 // Batch-shared dense integration (integrators.py:127-174), one chunk.
 struct DenseIO {
   double *y;            // [8][ld] in: states at the chunk's start; out: at its end
@@ -168,6 +187,8 @@ struct LaunchShape {
   hipError_t launch_rhs_vjp(const JobConst &c, const double *state, const double *v, \
                             int64_t n, double *zbar, double *abar, hipStream_t s); \
   hipError_t launch_integrate_dense(const JobConst &c, const DenseIO &d, hipStream_t s); \
+  hipError_t launch_trace_shader(const JobConst &c, const ShaderConst &sp, const ShaderIO &io, \
+                                 hipStream_t s);                                   \
   hipError_t launch_trace_tile(const JobConst &c, const TileConst &t, const TileIO &io, \
                                int block, hipStream_t s);                         \
   hipError_t launch_midpoint(const JobConst &c, double *y_soa, int64_t ld, const double *h, \

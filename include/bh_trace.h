@@ -250,6 +250,39 @@ int32_t bh_trace_tile_f64(const bh_trace_params *p, const bh_tile_params *t,
                           double *hit_mom, uint64_t *iterations, const int64_t *order,
                           int64_t n, void *stream);
 
+/* ---- the real-time shader's algorithm ---------------------------------
+ * trace_like_shader (realtime/reference.py:41-188): fixed RK4 steps,
+ * h = clip(step_scale (r - r_plus), min_step, max_step) (interior cameras:
+ * step_scale min(max(|r - r_plus|, 0.15 r), 0.5 r)), then
+ * h = min(h, 1 / max(1, |p|)); each pass first classifies: capture at
+ * r <= capture_radius or |p|^2 >= momentum_bailout^2 (interior: status
+ * TERMINATED, also r <= 1.002 capture_radius with |p|^2 > 2500), escape at
+ * r >= escape_radius; an equatorial crossing interpolated linearly inside
+ * [disk_inner, disk_outer] ends the ray BH_DISK; rays left after max_steps
+ * passes stay BH_MAX_STEPS. */
+typedef struct bh_shader_params {
+  double mass;
+  double spin;
+  double capture_radius;  /* r_plus (1 + capture_margin), or the interior stop */
+  double escape_radius;
+  double step_scale;
+  double min_step;
+  double max_step;
+  double momentum_bailout;
+  double disk_inner;
+  double disk_outer;
+  int64_t max_steps;      /* the caller applies the interior cap of 1024 */
+  int32_t flags;          /* BH_TILE_DISK | BH_TILE_INTERIOR */
+  int32_t mode;
+} bh_shader_params;
+
+/* Outputs: final states (SoA, NULL to skip), status, and optional hit
+ * (n, 8) = (crossing radius, position (3), momentum (4)) of disk hits, NaN
+ * otherwise.  order as bh_trace_plain_f64. */
+int32_t bh_trace_shader_f64(const bh_shader_params *p, const double *state_in_soa, int64_t ld_in,
+                            double *state_out_soa, int64_t ld_out, int32_t *status, double *hit,
+                            const int64_t *order, int64_t n, void *stream);
+
 /* rk4_step (integrators.py:57-73) with the geodesic RHS, applied n_steps
  * times per ray (the reference's fixed-step throughput benchmark,
  * examples/benchmark.py:34-62, loops rk4_step n_steps times): in place on

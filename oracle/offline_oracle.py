@@ -6,16 +6,21 @@ classification before each step and equatorial disk crossings
 (offline/render.py:184-309), the crossing refinement by bisection along
 RK4 sub-steps (render.py:312-349), the flat-space escape direction
 (render.py:164-181), the photon-shell retrace of saturated rays
-(render.py:612-630) and the camera sample grid (render.py:101-162,
-realtime/fly_camera.py:27-107).  Only tests, smoke() and the CPU-baseline
-leg of bench.py use it, as the checker; the product never imports it.
+(render.py:612-630), the camera sample grid (render.py:101-162,
+realtime/fly_camera.py:27-107) and the real-time shader's fixed-step
+algorithm (realtime/reference.py:41-188).  Only tests, smoke() and the
+CPU-baseline leg of bench.py use it, as the checker; the product never
+imports it.
 
 Pinning (tests/test_oracle_offline.py): Schwarzschild face-on disk hits
 against an independent quadrature of the orbit equation; crossings at
 z = 0 inside the annulus; the trace loop without a disk against the
 tracer restatement (bh_oracle.trace_rays) where the two loops must agree;
 flat directions of rays escaping radially; saturated/retrace bookkeeping
-on constructed cases.
+on constructed cases; for the shader algorithm the reference's own accuracy
+checks (every preset classifies an a = 0.9 fan as the adaptive tracer does,
+b_c = 3 sqrt(3) within 0.5%, no numerical failures) and disk hits on the
+crossing chord.
 """
 
 from __future__ import annotations
@@ -79,7 +84,8 @@ class FlyCamera:
 
 
 def pinhole_grid(width, height, fov_degrees, supersample):
-    """render.py:101-144 (vertical FOV; pixel row, column, subpixel row, column)."""
+    """render.py:101-144 (tan(fov / 2) scales the horizontal coordinate;
+    order: pixel row, column, subpixel row, column)."""
     tan_half = math.tan(math.radians(fov_degrees) / 2.0)
     aspect = width / height
     ss = supersample
@@ -237,3 +243,111 @@ def trace_tile_retrace(st: Kerr, state0, escape_radius, disk_radii, rtol, refine
         result.hit_positions[retrace] = refined.hit_positions
         result.hit_momenta[retrace] = refined.hit_momenta
     return result, retrace
+
+
+# ---------------------------------------------------------------------------
+# the real-time shader's algorithm (realtime/reference.py:41-188)
+# ---------------------------------------------------------------------------
+
+PRESETS = {  # realtime/settings.py:33-50 (max_steps, step_scale)
+    "low": (256, 0.35), "medium": (512, 0.22), "high": (1024, 0.14), "ultra": (2048, 0.08)}
+
+
+@dataclass
+class ShaderOutput:
+    status: np.ndarray
+    positions: np.ndarray
+    momenta: np.ndarray
+    hit_radii: np.ndarray | None
+    hit_positions: np.ndarray | None
+    hit_momenta: np.ndarray | None
+
+
+def trace_like_shader(st: Kerr, state0, escape_radius, max_steps=512, step_scale=0.22,
+                      min_step=1e-4, max_step=4.0, capture_margin=2e-2, momentum_bailout=1e3,
+                      disk_radii=None, interior_stop=None):
+    """realtime/reference.py:41-188 with the RenderSettings fields as arguments."""
+    r_plus = st.outer_horizon_radius
+    if interior_stop is None:
+        capture_radius = r_plus * (1.0 + capture_margin)
+        capture_status = int(RayStatus.CAPTURED)
+    else:
+        capture_radius = interior_stop
+        capture_status = int(RayStatus.TERMINATED)
+    n = state0.shape[0]
+    state = state0.copy()
+    status = np.full((n,), int(RayStatus.MAX_STEPS), dtype=np.int32)
+    active = np.ones((n,), dtype=bool)
+    hit_radii = np.full((n,), np.nan)
+    hit_positions = np.full((n, 3), np.nan)
+    hit_momenta = np.full((n, 4), np.nan)
+    if interior_stop is not None:
+        max_steps = min(max_steps, 1024)
+    for _ in range(max_steps):
+        idx = np.nonzero(active)[0]
+        if idx.size == 0:
+            break
+        y = state[idx]
+        radius = st.kerr_schild_radius(y[:, 1], y[:, 2], y[:, 3])
+        momentum2 = np.sum(y[:, 5:8] * y[:, 5:8], axis=-1)
+        captured = (radius <= capture_radius) | (momentum2 >= momentum_bailout ** 2)
+        if interior_stop is not None:
+            captured = captured | ((radius <= capture_radius * 1.002) & (momentum2 > 2500.0))
+        escaped = ~captured & (radius >= escape_radius)
+        status[idx[captured]] = capture_status
+        status[idx[escaped]] = int(RayStatus.ESCAPED)
+        active[idx] = ~(captured | escaped)
+        keep = ~(captured | escaped)
+        idx = idx[keep]
+        if idx.size == 0:
+            break
+        y = y[keep]
+        radius = radius[keep]
+        if interior_stop is None:
+            scale_length = radius - r_plus
+        else:
+            scale_length = np.minimum(np.maximum(np.abs(radius - r_plus), 0.15 * radius),
+                                      0.5 * radius)
+        h = np.clip(step_scale * scale_length, min_step, max_step)
+        h = np.minimum(h, 1.0 / np.maximum(1.0, np.sqrt(momentum2[keep])))
+        hh = h[:, None]
+        k1 = geodesic_rhs(st, y)
+        k2 = geodesic_rhs(st, y + 0.5 * hh * k1)
+        k3 = geodesic_rhs(st, y + 0.5 * hh * k2)
+        k4 = geodesic_rhs(st, y + hh * k3)
+        y_new = y + (hh / 6.0) * (k1 + 2.0 * k2 + 2.0 * k3 + k4)
+        if disk_radii is not None:
+            crossed = y[:, 3] * y_new[:, 3] < 0.0
+            if bool(crossed.any()):
+                t_cross = y[crossed, 3] / (y[crossed, 3] - y_new[crossed, 3])
+                interpolated = y[crossed] + t_cross[:, None] * (y_new[crossed] - y[crossed])
+                r_hit = st.kerr_schild_radius(interpolated[:, 1], interpolated[:, 2],
+                                              interpolated[:, 3])
+                on_disk = (r_hit >= disk_radii[0]) & (r_hit <= disk_radii[1])
+                hit_local = np.zeros(y.shape[0], dtype=bool)
+                hit_local[np.nonzero(crossed)[0][on_disk]] = True
+                targets = idx[hit_local]
+                status[targets] = int(RayStatus.DISK)
+                active[targets] = False
+                hit_radii[targets] = r_hit[on_disk]
+                hit_positions[targets] = interpolated[on_disk][:, 1:4]
+                hit_momenta[targets] = interpolated[on_disk][:, 4:8]
+        state[idx] = y_new
+    disk = disk_radii is not None
+    return ShaderOutput(status=status, positions=state[:, 1:4].copy(),
+                        momenta=state[:, 4:8].copy(), hit_radii=hit_radii if disk else None,
+                        hit_positions=hit_positions if disk else None,
+                        hit_momenta=hit_momenta if disk else None)
+
+
+def equatorial_fan(st: Kerr, offsets, distance=30.0):
+    """test_realtime_reference.py:25-38: past-directed equatorial rays from
+    (D, 0, 0) along (-1, offset, 0); returns (states, impact parameters)."""
+    n = offsets.shape[0]
+    positions = np.tile([[distance, 0.0, 0.0]], (n, 1))
+    directions = np.stack([-np.ones(n), offsets, np.zeros(n)], axis=-1)
+    directions /= np.linalg.norm(directions, axis=-1, keepdims=True)
+    from oracle.bh_oracle import build_state, conserved_quantities, null_momentum_from_velocity
+    state0 = build_state(positions, null_momentum_from_velocity(st, positions, directions, "past"))
+    energy, l_z = conserved_quantities(state0)
+    return state0, l_z / energy

@@ -140,3 +140,55 @@ def test_subpixel_grid_geometry():
     np.testing.assert_allclose(tan_el, (1.0 - 0.25 / 6 * 2) * math.tan(math.radians(35.0))
                                / (8 / 6), rtol=1e-12)
     assert np.dot(cam.basis()[0], fwd) > 1.0 - 1e-12
+
+
+# ---------------------------------------------------------------------------
+# the real-time shader's algorithm: the reference's own accuracy checks
+# (test_realtime_reference.py:56-129) applied to the oracle
+# ---------------------------------------------------------------------------
+
+def _black(status):
+    return (status == 1) | (status == 3)
+
+
+def test_shader_presets_match_adaptive_tracer_for_kerr():
+    st = orc.Kerr(spin=0.9)
+    offsets = np.linspace(0.03, 0.40, 300)
+    s0, _ = off.equatorial_fan(st, np.concatenate([offsets, -offsets]))
+    truth = orc.trace_rays(st, s0, escape_radius=60.0, rtol=1e-10)
+    for name, (max_steps, scale) in off.PRESETS.items():
+        ref = off.trace_like_shader(st, s0, 60.0, max_steps=max_steps, step_scale=scale)
+        assert np.array_equal(_black(ref.status), _black(truth.status)), name
+
+
+def test_shader_schwarzschild_boundary_medium():
+    st = orc.Kerr(spin=0.0)
+    s0, impact = off.equatorial_fan(st, np.linspace(0.10, 0.25, 600))
+    ref = off.trace_like_shader(st, s0, 60.0, *off.PRESETS["medium"])
+    black = _black(ref.status)
+    b = 0.5 * (np.abs(impact)[black].max() + np.abs(impact)[~black].min())
+    assert abs(b - 3.0 * np.sqrt(3.0)) / (3.0 * np.sqrt(3.0)) < 5e-3
+
+
+def test_shader_plungers_never_fail_numerically():
+    st = orc.Kerr(spin=0.998)
+    s0, _ = off.equatorial_fan(st, np.linspace(0.001, 0.15, 200))
+    ref = off.trace_like_shader(st, s0, 60.0, *off.PRESETS["low"])
+    assert np.all(np.isfinite(ref.positions))
+    assert np.all(ref.status == 1)
+
+
+def test_shader_disk_interpolation_lies_on_the_chord():
+    """Disk hits are the linear interpolation of the crossing step: z = 0 at
+    the hit and the hit radius inside the annulus."""
+    st = orc.Kerr(spin=0.9)
+    cam = off.FlyCamera.from_orbit(26.0, 80.0)
+    dirs = off.subpixel_directions(cam, 24, 16, 60.0, 1)
+    pos = np.tile(cam.position, (len(dirs), 1))
+    s0 = orc.build_state(pos, orc.null_momentum_from_velocity(st, pos, dirs))
+    res = off.trace_like_shader(st, s0, 60.0, disk_radii=(st.isco_radius(True), 18.0))
+    hit = res.status == 5
+    assert hit.sum() > 20
+    assert np.max(np.abs(res.hit_positions[hit, 2])) < 1e-12
+    assert np.all((res.hit_radii[hit] >= st.isco_radius(True)) & (res.hit_radii[hit] <= 18.0))
+    assert np.all(np.isnan(res.hit_radii[~hit]))
