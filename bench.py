@@ -6,8 +6,9 @@ One bench step traces the whole 1920x1080 sweep -- spins a in {0, 0.5, 0.9,
 0.998} x inclinations {17, 80} deg, camera at r = 50 M, horizontal FOV 25 deg,
 rtol 1e-8 / atol 1e-10, capture at r_+ + 0.01 M, escape at 100 M, max_steps
 20000 -- one plain-tracer launch per frame (8 launches, 16.6 M rays), rays
-handed to wavefront lanes in 8x8 pixel tiles (--order; results do not depend
-on it).  Rays are generated on the device before timing (camera directions
+handed to wavefront lanes in 8x8 pixel tiles, the tiles nearest the image
+centre (shadow and photon ring, the longest traces) first (--order; results
+do not depend on it).  Rays are generated on the device before timing (camera directions
 on the host, null momenta and the SoA layout on the device), so the timed
 region is the tracer kernels only, with inputs resident in HBM.
 
@@ -128,6 +129,8 @@ def camera_order(job, order: str):
                                    azimuth_degrees=job.azimuth_degrees,
                                    fov_degrees=job.fov_degrees, width=job.width,
                                    height=job.height)
+    if order.startswith("center"):
+        return cam.center_tile_order(int(order[6:]) if order[6:] else 8)
     return cam.tile_order(int(order[4:]) if order.startswith("tile") and order[4:] else 8)
 
 
@@ -627,8 +630,9 @@ def main(argv=None) -> int:
     ap.add_argument("--tune", default="auto",
                     help='launch shape: "auto" (autotuner, untimed, before the sweep), '
                          '"default" (256,1,1) or "block,rays_per_lane,refill_min"')
-    ap.add_argument("--order", default="tile8",
-                    help="ray hand-out order: row (pixel order) or tile<k> (k x k pixel tiles)")
+    ap.add_argument("--order", default="center8",
+                    help="ray hand-out order: row (pixel order), tile<k> (k x k pixel tiles, "
+                         "row-major) or center<k> (k x k tiles, nearest the image centre first)")
     ap.add_argument("--cpu-rays-per-frame", type=int, default=6144)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-adjoint", action="store_true",

@@ -127,3 +127,26 @@ class PinholeCamera:
             + (r % tile) * tile + (c % tile)
         return numpy.argsort(key.ravel(), kind="stable").astype(numpy.int64)
 
+    def center_tile_order(self, tile: int = 8) -> numpy.ndarray:
+        """tile_order with the tiles sorted by the distance of their centre
+        from the image centre, nearest first (ties in row-major order).  A
+        camera aimed at the hole sees the shadow and the photon ring -- the
+        longest traces -- around the image centre, so their wavefronts are
+        dispatched first and the kernel's tail is made of short rays (the
+        longest-job-first rule, with no trace needed to rank the tiles)."""
+        if tile < 1:
+            raise ValueError("tile must be >= 1")
+        tiles_per_row = (self.width + tile - 1) // tile
+        tiles_per_col = (self.height + tile - 1) // tile
+        ty, tx = numpy.meshgrid(numpy.arange(tiles_per_col), numpy.arange(tiles_per_row),
+                                indexing="ij")
+        d = numpy.hypot((ty + 0.5) * tile - self.height / 2.0,
+                        (tx + 0.5) * tile - self.width / 2.0).ravel()
+        rank = numpy.argsort(d, kind="stable")
+        rank_of = numpy.empty_like(rank)
+        rank_of[rank] = numpy.arange(rank.size)
+        r, c = numpy.meshgrid(numpy.arange(self.height), numpy.arange(self.width), indexing="ij")
+        tid = (r // tile) * tiles_per_row + (c // tile)
+        key = rank_of[tid] * (tile * tile) + (r % tile) * tile + (c % tile)
+        return numpy.argsort(key.ravel(), kind="stable").astype(numpy.int64)
+

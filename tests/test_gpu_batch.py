@@ -136,5 +136,7 @@ def test_tile_order_bitwise_identical(hip, tile):
     a = hip.trace_rays(st, s0, **_job_kw(job))
     b = hip.trace_rays(st, s0, ray_order=order, **_job_kw(job))
     _assert_same(a, b, f"tile {tile}")
+    c = hip.trace_rays(st, s0, ray_order=cam.center_tile_order(tile), **_job_kw(job))
+    _assert_same(a, c, f"centre-first tile {tile}")
     with pytest.raises(ValueError):
         hip.trace_rays(st, s0, ray_order=np.zeros(job.n_rays, dtype=np.int64), **_job_kw(job))

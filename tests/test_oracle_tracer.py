@@ -175,3 +175,13 @@ def test_camera_tile_order_is_permutation():
             first = o[:64]
             assert np.array_equal(np.unique(first // 160), np.arange(8))
             assert np.array_equal(np.unique(first % 160), np.arange(8))
+        c = cam.center_tile_order(t)
+        assert np.array_equal(np.sort(c), np.arange(w * h))
+        if w == 160:
+            # the first tile is one of the four around the image centre; the
+            # last ones are corner tiles
+            first = c[:64]
+            assert set(np.unique(first // 160)) <= set(range(52, 68))
+            assert set(np.unique(first % 160)) <= set(range(72, 88))
+            last = c[-1]
+            assert last // 160 in range(0, 8) or last // 160 in range(112, 120)
