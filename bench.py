@@ -791,6 +791,8 @@ def main(argv=None) -> int:
                       + tracer.resolve_mode(args.mode, jobs[0].rtol) + " build)",
             "flop_per_launch": flop / len(events),
             "mean_launch_ms": float(np.mean(kern_ms)),
+            "kernel_resources": _lib.kernel_attributes(
+                "plain", tracer.resolve_mode(args.mode, jobs[0].rtol), tune.block_size),
         },
         "cpu_baseline": None if cpu is None else {
             k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},

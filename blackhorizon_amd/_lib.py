@@ -117,6 +117,7 @@ SIGNATURES = {
                           _P, _P, _P, _I64, _P],
     "bh_camera_rays_f64": [_D, _D, _P, _D, _D, _I64, _I64, _I32, _P, _I64, _P],
     "bh_shade_checker_u8": [_D, _D, _P, _I64, _P, _I64, _I32, _P, _P],
+    "bh_kernel_attributes": [_I32, _I32, _I32, _P],
     "bh_trace_shader_f64": [ctypes.POINTER(ShaderParams), _P, _I64, _P, _I64, _P, _P, _P, _I64,
                             _P],
     "bh_midpoint_f64": [_D, _D, _I32, _P, _I64, _P, _I64, _I64, _I32, _P],
@@ -190,6 +191,21 @@ def require_device() -> ctypes.CDLL:
 def stream_handle(stream=None) -> int:
     s = stream if stream is not None else torch.cuda.current_stream()
     return int(s.cuda_stream)
+
+
+KERNELS = {"plain": 0, "batched": 1, "record": 2, "adjoint": 3, "tile": 4, "shader": 5}
+
+
+def kernel_attributes(kernel: str, mode: str = "fast", block: int = 256) -> dict:
+    """Registers, static LDS and spill scratch per thread of one tracing
+    kernel, and its resident waves per SIMD at ``block`` threads per block
+    (bh_kernel_attributes)."""
+    lib = require_device()
+    out = (ctypes.c_int32 * 4)()
+    check(lib.bh_kernel_attributes(KERNELS[kernel], MODE_FAST if mode == "fast" else MODE_PRECISE,
+                                   block, ctypes.addressof(out)), "kernel_attributes")
+    return dict(vgprs=out[0], lds_bytes=out[1], scratch_bytes=out[2], blocks_per_cu=out[3],
+                waves_per_simd=out[3] * (block // 64) / 4.0)
 
 
 def ptr(t) -> int | None:

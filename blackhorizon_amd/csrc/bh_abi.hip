@@ -308,6 +308,17 @@ int32_t bh_dp5_step_f64(double mass, double spin, int32_t mode, double rtol, dou
   return hip_status(e, "bh_dp5_step_f64");
 }
 
+int32_t bh_kernel_attributes(int32_t kernel, int32_t mode, int32_t block, int32_t *out) {
+  if (!out) return fail(BH_EINVAL, "out is NULL");
+  if (mode != BH_MODE_FAST && mode != BH_MODE_PRECISE) return fail(BH_EINVAL, "bad mode");
+  if (kernel < BH_KERNEL_PLAIN || kernel > BH_KERNEL_SHADER) return fail(BH_EINVAL, "bad kernel");
+  if (block <= 0 || block > 256 || block % 64) return fail(BH_EINVAL, "bad block");
+  if (kernel == BH_KERNEL_ADJOINT && block != 64) return fail(BH_EINVAL, "the adjoint runs 64-thread blocks");
+  hipError_t e = mode == BH_MODE_PRECISE ? precise::kernel_attributes(kernel, block, out)
+                                         : fast::kernel_attributes(kernel, block, out);
+  return hip_status(e, "bh_kernel_attributes");
+}
+
 int32_t bh_trace_shader_f64(const bh_shader_params *p, const double *state_in_soa, int64_t ld_in,
                             double *state_out_soa, int64_t ld_out, int32_t *status, double *hit,
                             const int64_t *order, int64_t n, void *stream) {

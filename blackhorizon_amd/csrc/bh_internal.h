@@ -187,6 +187,7 @@ struct LaunchShape {
   hipError_t launch_rhs_vjp(const JobConst &c, const double *state, const double *v, \
                             int64_t n, double *zbar, double *abar, hipStream_t s); \
   hipError_t launch_integrate_dense(const JobConst &c, const DenseIO &d, hipStream_t s); \
+  hipError_t kernel_attributes(int which, int block, int32_t *out);                \
   hipError_t launch_trace_shader(const JobConst &c, const ShaderConst &sp, const ShaderIO &io, \
                                  hipStream_t s);                                   \
   hipError_t launch_trace_tile(const JobConst &c, const TileConst &t, const TileIO &io, \

@@ -88,6 +88,18 @@ typedef struct bh_tune_params {
 int32_t bh_abi_version(void);
 const char *bh_last_error(void);
 
+/* Resource usage of one tracing kernel of one arithmetic build, for reports:
+ * out[0] registers per thread (VGPRs), out[1] static LDS bytes per block,
+ * out[2] private scratch bytes per thread (spills), out[3] resident blocks
+ * per CU at `block` threads per block.  No reference counterpart. */
+#define BH_KERNEL_PLAIN 0
+#define BH_KERNEL_BATCHED 1
+#define BH_KERNEL_RECORD 2
+#define BH_KERNEL_ADJOINT 3
+#define BH_KERNEL_TILE 4
+#define BH_KERNEL_SHADER 5
+int32_t bh_kernel_attributes(int32_t kernel, int32_t mode, int32_t block, int32_t *out);
+
 /* Number of rays with r0 >= escape_radius, written to *count_out (device
  * int32).  Replaces the initial-radius check of tracer.py:94-99; the host
  * raises ValueError when the count is nonzero. */

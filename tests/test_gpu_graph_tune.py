@@ -136,3 +136,17 @@ def test_batch_graph_replays_match_eager(hip):
 
 def _group_snapshot(gr):
     return [t.clone() for t in (gr.soa_out, gr.status, gr.steps, gr.attempts, gr.iters)]
+
+
+def test_kernel_attributes_report_resources(hip):
+    from blackhorizon_amd import _lib
+
+    for kernel in ("plain", "batched", "record", "tile", "shader"):
+        for mode in ("fast", "precise"):
+            a = _lib.kernel_attributes(kernel, mode, 256)
+            assert 32 <= a["vgprs"] <= 512, (kernel, mode, a)
+            assert a["blocks_per_cu"] >= 1 and a["waves_per_simd"] >= 1
+    a = _lib.kernel_attributes("adjoint", "fast", 64)
+    assert a["lds_bytes"] == 20480 and a["waves_per_simd"] >= 1
+    with pytest.raises(ValueError):
+        _lib.kernel_attributes("adjoint", "fast", 256)
