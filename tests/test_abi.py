@@ -79,3 +79,31 @@ def test_python_api_errors_without_device(monkeypatch):
         bh.get_xp("hip")
     with pytest.raises(ValueError):
         bh.get_xp("cuda")
+
+
+def test_new_entries_reject_bad_arguments():
+    """The offline, shader, dense, fixed-step, camera, shading and resource
+    entries validate before any device work."""
+    lib = _lib.load()
+    E = _lib.BH_EINVAL
+    tile = _lib.TileParams(6.0, 2.0, 10, _lib.TILE_DISK, 0)  # inner > outer
+    assert lib.bh_trace_tile_f64(_params(), tile, None, None, 0, None, 0, None, None, None, None,
+                                 None, None, None, None, None, 0, None) == E
+    assert lib.bh_trace_tile_f64(_params(), _lib.TileParams(2.0, 6.0, 10, 8, 0), None, None, 0,
+                                 None, 0, None, None, None, None, None, None, None, None, None, 0,
+                                 None) == E  # unknown flag
+    assert lib.bh_trace_tile_f64(_params(), None, None, None, 0, None, 0, None, None, None, None,
+                                 None, None, None, None, None, 0, None) == E
+    sp = _lib.ShaderParams(1.0, 0.9, 1.5, 60.0, 0.22, 1e-4, 4.0, 1e3, 0.0, 0.0, 512, 0, 9)
+    assert lib.bh_trace_shader_f64(sp, None, 0, None, 0, None, None, None, 0, None) == E
+    assert lib.bh_integrate_dense_f64(1.0, 0.9, 1, 1e-10, 1e-12, 2.0, 1.0, None, 0, 0, None, None,
+                                      1, None, None, None, None) == E  # n = 0
+    assert lib.bh_rk4_f64(1.0, 0.9, 1, None, 0, None, 4, 1, None) == E  # ld < n
+    assert lib.bh_midpoint_f64(1.0, 0.9, 1, None, 4, None, 4, 1, -1, None) == E
+    assert lib.bh_camera_rays_f64(1.0, 0.9, None, 0.1, 0.1, 4, 4, 1, None, 16, None) == E
+    assert lib.bh_shade_checker_u8(1.0, 0.9, None, 4, None, 4, 0, None, None) == E  # cells 0
+    out = (ctypes.c_int32 * 4)()
+    assert lib.bh_kernel_attributes(9, 0, 256, ctypes.addressof(out)) == E
+    assert lib.bh_kernel_attributes(0, 0, 96, ctypes.addressof(out)) == E
+    assert lib.bh_adjoint_scratch_doubles(0) == -1
+    assert lib.bh_adjoint_scratch_doubles(32) == 9 * 32
