@@ -577,6 +577,8 @@ def run_orbits(args) -> dict:
     s1 = torch.cat([r.states for r in res])
     status = torch.cat([r.status for r in res])
     steps = int(sum(int(r.steps.sum().item()) for r in res))
+    per_orbit = torch.cat([r.steps for r in res]).double()
+    att = torch.cat([r.attempts for r in res]).double()
     d0 = geodesics._diagnostics(st, s0, ("ham", "energy", "lz"))
     d1 = geodesics._diagnostics(st, s1, ("ham", "energy", "lz"))
     ok = status == 6
@@ -587,7 +589,14 @@ def run_orbits(args) -> dict:
                status_counts=torch.bincount(status.long(), minlength=7).tolist(),
                energy_drift_max=float((d1["energy"] - d0["energy"]).abs().max()),
                lz_rel_drift_max=float(dlz.max()), lz_rel_drift_median=float(dlz.median()),
-               ham_drift_max=float(dh.max()), ham_drift_median=float(dh.median()))
+               ham_drift_max=float(dh.max()), ham_drift_median=float(dh.median()),
+               # the launch cannot end before its longest orbit: attempts of
+               # that orbit against the mean show how much of the time is tail
+               steps_per_orbit={"median": float(per_orbit.median()),
+                                "p99": float(torch.quantile(per_orbit, 0.99)),
+                                "max": float(per_orbit.max())},
+               attempts_total=int(att.sum().item()), attempts_max=int(att.max().item()),
+               ns_per_attempt_of_longest=ms * 1e6 / float(att.max()))
     # adjoint of the final positions (one recorded launch over all orbits)
     cfg = adjoint.TraceSettings(escape_radius=1000.0, max_steps=10_000_000, rtol=1e-10,
                                 atol=1e-12, initial_step=0.05, min_step=1e-14, K=32,

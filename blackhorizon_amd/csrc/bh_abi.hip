@@ -73,6 +73,20 @@ static int32_t check_spacetime(double mass, double spin) {
   return BH_OK;
 }
 
+// Largest ray count an entry accepts: 2^36 rays keep every grid below 2^32
+// workgroups down to 16-thread blocks (one SoA plane of that many rays is
+// 512 GiB, beyond one GPU's memory), and ray-count arithmetic far from int64
+// overflow.
+constexpr int64_t MAX_RAYS = (int64_t)1 << 36;
+
+static bool rays_ok(int64_t n) { return n >= 0 && n <= MAX_RAYS; }
+
+static int32_t check_mode(int32_t mode) {
+  if (mode != BH_MODE_FAST && mode != BH_MODE_PRECISE)
+    return fail(BH_EINVAL, "mode must be BH_MODE_FAST or BH_MODE_PRECISE");
+  return BH_OK;
+}
+
 static int32_t check_params(const bh_trace_params *p) {
   if (!p) return fail(BH_EINVAL, "params is NULL");
   int32_t rc = check_spacetime(p->mass, p->spin);
@@ -105,7 +119,7 @@ int32_t bh_count_outside_f64(const bh_trace_params *p, const double *state_soa, 
                              int64_t n, int32_t *count_out, void *stream) {
   int32_t rc = check_params(p);
   if (rc) return rc;
-  if (n < 0 || ld < n) return fail(BH_EINVAL, "bad n / ld");
+  if (!rays_ok(n) || ld < n) return fail(BH_EINVAL, "bad n / ld");
   if (!count_out || (n > 0 && !state_soa)) return fail(BH_EINVAL, "NULL buffer");
   hipStream_t s = (hipStream_t)stream;
   JobConst c = make_job_const(*p);
@@ -123,7 +137,7 @@ int32_t bh_trace_plain_f64(const bh_trace_params *p, const bh_tune_params *tune,
                            void *stream) {
   int32_t rc = check_params(p);
   if (rc) return rc;
-  if (n < 0 || ld_in < n || ld_out < n) return fail(BH_EINVAL, "bad n / ld");
+  if (!rays_ok(n) || ld_in < n || ld_out < n) return fail(BH_EINVAL, "bad n / ld");
   if (n > 0 && (!state_in_soa || !state_out_soa || !status || !steps || !attempts))
     return fail(BH_EINVAL, "NULL buffer");
   LaunchShape shape{256, 1, 1};
@@ -159,7 +173,7 @@ int32_t bh_trace_batched_f64(const bh_trace_params *jobs, const int64_t *ray_off
     return fail(BH_EINVAL, "n_jobs must be in [1, 256]");
   if (mode != BH_MODE_FAST && mode != BH_MODE_PRECISE)
     return fail(BH_EINVAL, "mode must be BH_MODE_FAST or BH_MODE_PRECISE");
-  if (n_total < 0 || ld_in < n_total || ld_out < n_total) return fail(BH_EINVAL, "bad n / ld");
+  if (!rays_ok(n_total) || ld_in < n_total || ld_out < n_total) return fail(BH_EINVAL, "bad n / ld");
   if (!jobs || !ray_offsets || !iterations || !workspace) return fail(BH_EINVAL, "NULL buffer");
   if (n_total > 0 && (!state_in_soa || !state_out_soa || !status || !steps || !attempts))
     return fail(BH_EINVAL, "NULL buffer");
@@ -199,7 +213,7 @@ int32_t bh_trace_record_f64(const bh_trace_params *p, const double *state_in_soa
                             int64_t *ev_step, double *theta, int32_t *event, void *stream) {
   int32_t rc = check_params(p);
   if (rc) return rc;
-  if (n < 0 || ld_in < n || ld_out < n) return fail(BH_EINVAL, "bad n / ld");
+  if (!rays_ok(n) || ld_in < n || ld_out < n) return fail(BH_EINVAL, "bad n / ld");
   if (K < 1 || K > MAX_ADJOINT_K || cap < 1)
     return fail(BH_EINVAL, "need 1 <= K <= 2^20 and cap >= 1");
   if (n > 0 && (!state_in_soa || !state_out_soa || !status || !steps || !attempts || !ckpt ||
@@ -225,7 +239,7 @@ int32_t bh_trace_adjoint_f64(const bh_trace_params *p, int64_t n, int32_t K, dou
                              const int64_t *order, void *stream) {
   int32_t rc = check_params(p);
   if (rc) return rc;
-  if (n < 0) return fail(BH_EINVAL, "bad n");
+  if (!rays_ok(n)) return fail(BH_EINVAL, "bad n");
   if (K < 1 || K > MAX_ADJOINT_K) return fail(BH_EINVAL, "need 1 <= K <= 2^20");
   if (n > 0 && (!ckpt || !event || !ev_step || !evy1 || !prev || !theta || !gterm || !gy0 ||
                 !gspin || !scratch))
@@ -243,7 +257,7 @@ int32_t bh_rhs_vjp_f64(double mass, double spin, const double *state, const doub
                        int64_t n, double *zbar, double *abar, void *stream) {
   int32_t rc = check_spacetime(mass, spin);
   if (rc) return rc;
-  if (n < 0) return fail(BH_EINVAL, "bad n");
+  if (!rays_ok(n)) return fail(BH_EINVAL, "bad n");
   if (n > 0 && (!state || !v || !zbar || !abar)) return fail(BH_EINVAL, "NULL buffer");
   JobConst c = spacetime_const(mass, spin);
   return hip_status(fast::launch_rhs_vjp(c, state, v, n, zbar, abar, (hipStream_t)stream),
@@ -251,13 +265,13 @@ int32_t bh_rhs_vjp_f64(double mass, double spin, const double *state, const doub
 }
 
 int32_t bh_aos_to_soa_f64(const double *aos, double *soa, int64_t ld, int64_t n, void *stream) {
-  if (n < 0 || ld < n) return fail(BH_EINVAL, "bad n / ld");
+  if (!rays_ok(n) || ld < n) return fail(BH_EINVAL, "bad n / ld");
   if (n > 0 && (!aos || !soa)) return fail(BH_EINVAL, "NULL buffer");
   return hip_status(launch_aos_to_soa(aos, soa, ld, n, (hipStream_t)stream), "bh_aos_to_soa_f64");
 }
 
 int32_t bh_soa_to_aos_f64(const double *soa, int64_t ld, double *aos, int64_t n, void *stream) {
-  if (n < 0 || ld < n) return fail(BH_EINVAL, "bad n / ld");
+  if (!rays_ok(n) || ld < n) return fail(BH_EINVAL, "bad n / ld");
   if (n > 0 && (!aos || !soa)) return fail(BH_EINVAL, "NULL buffer");
   return hip_status(launch_soa_to_aos(soa, ld, aos, n, (hipStream_t)stream), "bh_soa_to_aos_f64");
 }
@@ -267,7 +281,8 @@ int32_t bh_geometry_f64(double mass, double spin, int32_t mode, const double *xy
                         void *stream) {
   int32_t rc = check_spacetime(mass, spin);
   if (rc) return rc;
-  if (n < 0) return fail(BH_EINVAL, "bad n");
+  if ((rc = check_mode(mode))) return rc;
+  if (!rays_ok(n)) return fail(BH_EINVAL, "bad n");
   if (n > 0 && !xyz) return fail(BH_EINVAL, "NULL buffer");
   JobConst c = spacetime_const(mass, spin);
   hipStream_t s = (hipStream_t)stream;
@@ -281,7 +296,8 @@ int32_t bh_geodesic_rhs_f64(double mass, double spin, int32_t mode, const double
                             int64_t n, double *deriv, void *stream) {
   int32_t rc = check_spacetime(mass, spin);
   if (rc) return rc;
-  if (n < 0) return fail(BH_EINVAL, "bad n");
+  if ((rc = check_mode(mode))) return rc;
+  if (!rays_ok(n)) return fail(BH_EINVAL, "bad n");
   if (n > 0 && (!state || !deriv)) return fail(BH_EINVAL, "NULL buffer");
   JobConst c = spacetime_const(mass, spin);
   hipStream_t s = (hipStream_t)stream;
@@ -295,7 +311,8 @@ int32_t bh_dp5_step_f64(double mass, double spin, int32_t mode, double rtol, dou
                         double *error, double *ratio, double *factor, void *stream) {
   int32_t rc = check_spacetime(mass, spin);
   if (rc) return rc;
-  if (n < 0) return fail(BH_EINVAL, "bad n");
+  if ((rc = check_mode(mode))) return rc;
+  if (!rays_ok(n)) return fail(BH_EINVAL, "bad n");
   if (n > 0 && (!state || !h || !new_state)) return fail(BH_EINVAL, "NULL buffer");
   JobConst c = spacetime_const(mass, spin);
   c.rtol = rtol;
@@ -328,7 +345,7 @@ int32_t bh_trace_shader_f64(const bh_shader_params *p, const double *state_in_so
   if (p->mode != BH_MODE_FAST && p->mode != BH_MODE_PRECISE) return fail(BH_EINVAL, "bad mode");
   if (p->max_steps < 0 || p->max_steps > ((int64_t)1 << 40)) return fail(BH_EINVAL, "bad max_steps");
   if (p->flags & ~(BH_TILE_DISK | BH_TILE_INTERIOR)) return fail(BH_EINVAL, "bad flags");
-  if (n < 0 || ld_in < n || (state_out_soa && ld_out < n)) return fail(BH_EINVAL, "bad n / ld");
+  if (!rays_ok(n) || ld_in < n || (state_out_soa && ld_out < n)) return fail(BH_EINVAL, "bad n / ld");
   if (n > 0 && (!state_in_soa || !status)) return fail(BH_EINVAL, "NULL buffer");
   bh_trace_params tp;
   memset(&tp, 0, sizeof(tp));
@@ -356,7 +373,7 @@ int32_t bh_midpoint_f64(double mass, double spin, int32_t mode, double *state_so
   int32_t rc = check_spacetime(mass, spin);
   if (rc) return rc;
   if (mode != BH_MODE_FAST && mode != BH_MODE_PRECISE) return fail(BH_EINVAL, "bad mode");
-  if (n < 0 || ld < n || n_steps < 0 || iterations < 0)
+  if (!rays_ok(n) || ld < n || n_steps < 0 || iterations < 0)
     return fail(BH_EINVAL, "bad n / ld / n_steps / iterations");
   if (n > 0 && (!state_soa || !h)) return fail(BH_EINVAL, "NULL buffer");
   JobConst c = spacetime_const(mass, spin);
@@ -378,7 +395,7 @@ int32_t bh_trace_tile_f64(const bh_trace_params *p, const bh_tile_params *t,
   int32_t rc = check_params(p);
   if (rc) return rc;
   if (!t) return fail(BH_EINVAL, "NULL tile parameters");
-  if (n < 0 || ld_in < n || (state_out_soa && ld_out < n)) return fail(BH_EINVAL, "bad n / ld");
+  if (!rays_ok(n) || ld_in < n || (state_out_soa && ld_out < n)) return fail(BH_EINVAL, "bad n / ld");
   if (n > 0 && (!state_in_soa || !status || !steps || !attempts || !saturated || !escape_dir ||
                 !hit_pos || !hit_mom))
     return fail(BH_EINVAL, "NULL buffer");
@@ -408,7 +425,7 @@ int32_t bh_rk4_f64(double mass, double spin, int32_t mode, double *state_soa, in
   int32_t rc = check_spacetime(mass, spin);
   if (rc) return rc;
   if (mode != BH_MODE_FAST && mode != BH_MODE_PRECISE) return fail(BH_EINVAL, "bad mode");
-  if (n < 0 || ld < n || n_steps < 0) return fail(BH_EINVAL, "bad n / ld / n_steps");
+  if (!rays_ok(n) || ld < n || n_steps < 0) return fail(BH_EINVAL, "bad n / ld / n_steps");
   if (n > 0 && (!state_soa || !h)) return fail(BH_EINVAL, "NULL buffer");
   JobConst c = spacetime_const(mass, spin);
   hipStream_t s = (hipStream_t)stream;
@@ -425,7 +442,7 @@ int32_t bh_integrate_dense_f64(double mass, double spin, int32_t mode, double rt
   int32_t rc = check_spacetime(mass, spin);
   if (rc) return rc;
   if (mode != BH_MODE_FAST && mode != BH_MODE_PRECISE) return fail(BH_EINVAL, "bad mode");
-  if (n < 1 || ld < n || cap < 1) return fail(BH_EINVAL, "need n >= 1, ld >= n, cap >= 1");
+  if (n < 1 || !rays_ok(n) || ld < n || cap < 1) return fail(BH_EINVAL, "need n >= 1, ld >= n, cap >= 1");
   if (!(rtol >= 0.0) || !(atol >= 0.0) || !(max_step > 0.0) || std::isnan(lambda_total))
     return fail(BH_EINVAL, "bad tolerances, max_step or lambda_total");
   if (!state_soa || !work || !traj || !ctl || !count || !flag)
@@ -446,7 +463,7 @@ int32_t bh_camera_rays_f64(double mass, double spin, const double *camera, doubl
   int32_t rc = check_spacetime(mass, spin);
   if (rc) return rc;
   if (!camera) return fail(BH_EINVAL, "camera is NULL");
-  if (width < 0 || height < 0 || (height > 0 && width > ((int64_t)1 << 40) / height))
+  if (width < 0 || height < 0 || (height > 0 && width > MAX_RAYS / height))
     return fail(BH_EINVAL, "bad width / height");
   if (ld < width * height) return fail(BH_EINVAL, "ld < width * height");
   if (width * height > 0 && !state_soa) return fail(BH_EINVAL, "NULL buffer");
@@ -470,7 +487,7 @@ int32_t bh_shade_checker_u8(double mass, double spin, const double *state_soa, i
                             void *stream) {
   int32_t rc = check_spacetime(mass, spin);
   if (rc) return rc;
-  if (n < 0 || ld < n || cells < 1) return fail(BH_EINVAL, "bad n / ld / cells");
+  if (!rays_ok(n) || ld < n || cells < 1) return fail(BH_EINVAL, "bad n / ld / cells");
   if (n > 0 && (!state_soa || !status || !rgb)) return fail(BH_EINVAL, "NULL buffer");
   JobConst c = spacetime_const(mass, spin);
   return hip_status(precise::launch_shade_checker(c, state_soa, ld, status, n, cells, rgb,
@@ -483,7 +500,7 @@ int32_t bh_null_momentum_f64(double mass, double spin, const double *positions,
                              int32_t normalize, double *momenta, void *stream) {
   int32_t rc = check_spacetime(mass, spin);
   if (rc) return rc;
-  if (n < 0) return fail(BH_EINVAL, "bad n");
+  if (!rays_ok(n)) return fail(BH_EINVAL, "bad n");
   if (n > 0 && (!positions || !directions || !momenta)) return fail(BH_EINVAL, "NULL buffer");
   JobConst c = spacetime_const(mass, spin);
   return hip_status(precise::launch_null_momentum(c, positions, directions, n, past ? 1 : 0,
@@ -496,7 +513,7 @@ int32_t bh_timelike_momentum_f64(double mass, double spin, const double *positio
                                  int32_t *bad_count, void *stream) {
   int32_t rc = check_spacetime(mass, spin);
   if (rc) return rc;
-  if (n < 0) return fail(BH_EINVAL, "bad n");
+  if (!rays_ok(n)) return fail(BH_EINVAL, "bad n");
   if (n > 0 && (!positions || !velocities || !momenta)) return fail(BH_EINVAL, "NULL buffer");
   JobConst c = spacetime_const(mass, spin);
   hipStream_t s = (hipStream_t)stream;
@@ -510,7 +527,7 @@ int32_t bh_diagnostics_f64(double mass, double spin, const double *state, int64_
                            double *energy, double *l_z, double *velocity, void *stream) {
   int32_t rc = check_spacetime(mass, spin);
   if (rc) return rc;
-  if (n < 0) return fail(BH_EINVAL, "bad n");
+  if (!rays_ok(n)) return fail(BH_EINVAL, "bad n");
   if (n > 0 && !state) return fail(BH_EINVAL, "NULL buffer");
   JobConst c = spacetime_const(mass, spin);
   return hip_status(

@@ -10,6 +10,8 @@
  *    NULL means the default stream).  No call synchronises the device.
  *  - Return value: BH_OK (0) or a negative BH_E* code.  bh_last_error()
  *    returns a thread-local message describing the last failure.
+ *  - Ray counts (n, width * height) are at most 2^36 per call; larger
+ *    counts, negative counts and ld < n are BH_EINVAL.
  *  - "state" layout at the boundary: rows (t, x, y, z, p_t, p_x, p_y, p_z)
  *    with covariant momenta (reference geodesics.py:3-8).  The tracer
  *    kernels take structure-of-arrays planes: component c of ray i lives at
