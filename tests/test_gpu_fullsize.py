@@ -1,0 +1,76 @@
+"""GPU: whole 1920x1080 frames of the large-trace configuration (2,073,600
+rays per launch), checked through what does not depend on the size:
+
+* the device-built camera states at the golden file's 96 seeded pixels equal
+  the oracle's bit for bit, and those pixels' results inside the full
+  launch meet the parity bar against the reference's own traces of them
+  (identical status and accepted steps, final states within 1e-7 or 10x
+  the reference's host-to-host spread);
+* the per-ray results of the whole frame are bitwise identical under
+  another hand-out order and launch shape (row-major pixels, 64-thread
+  blocks, in-wave refill) and a repeated launch;
+* p_t is carried exactly (E conserved bit for bit), every ray ends
+  captured or escaped, and the launch's iteration count is the largest
+  per-ray attempt count.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import golden
+from parity import assert_trace_parity, reference_spread
+from blackhorizon_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+# a = 0.9 at 17 deg (the headline spin) and a = 0.998 at 80 deg (the longest)
+FRAMES = [4, 7]
+
+
+@pytest.mark.parametrize("frame", FRAMES)
+def test_full_hd_frame(hip, frame):
+    from blackhorizon_amd.tracer import Tune
+
+    g = golden("hd_subsample.npz")
+    idx = g[f"f{frame}_index"]
+    job = workloads.hd_frames()[frame]
+    st = hip.KerrSpacetime(spin=job.spin)
+    cam = hip.PinholeCamera.from_orbit(job.distance, job.inclination_degrees,
+                                       fov_degrees=job.fov_degrees, width=job.width,
+                                       height=job.height)
+    s0 = cam.device_states(st)
+    assert s0.shape == (job.width * job.height, 8)
+    np.testing.assert_array_equal(s0[torch.from_numpy(idx).cuda()].cpu().numpy(),
+                                  g[f"f{frame}_state0"])
+    kw = dict(max_steps=job.max_steps, rtol=job.rtol, atol=job.atol,
+              capture_margin=job.capture_margin)
+    order = torch.from_numpy(cam.center_tile_order(8)).cuda()
+    res = hip.trace_rays(st, s0, job.escape_radius, ray_order=order, **kw)
+
+    # the sampled pixels of the full launch against the reference's traces
+    sel = torch.from_numpy(idx).cuda()
+    spread = reference_spread(job.spin, g[f"f{frame}_state0"], g[f"f{frame}_states"],
+                              escape_radius=job.escape_radius, **kw)
+    assert_trace_parity(res.states[sel].cpu().numpy(), res.status[sel].cpu().numpy(),
+                        res.steps[sel].cpu().numpy(), g[f"f{frame}_states"],
+                        g[f"f{frame}_status"], g[f"f{frame}_steps"],
+                        label=f"full frame {frame} sample", spread=spread)
+
+    # the whole frame: bitwise independent of hand-out order and launch shape
+    other = hip.trace_rays(st, s0, job.escape_radius, tune=Tune(64, 2, 16), **kw)
+    again = hip.trace_rays(st, s0, job.escape_radius, ray_order=order, **kw)
+    for r2 in (other, again):
+        assert torch.equal(r2.status, res.status)
+        assert torch.equal(r2.steps, res.steps)
+        assert torch.equal(r2.attempts, res.attempts)
+        assert torch.equal(r2.states.view(torch.int64), res.states.view(torch.int64))
+        assert r2.iterations == res.iterations
+
+    # conservation and termination
+    assert torch.equal(res.states[:, 4].view(torch.int64), s0[:, 4].view(torch.int64))
+    counts = torch.bincount(res.status.long(), minlength=7).cpu().numpy()
+    assert counts[1] + counts[2] == job.width * job.height, counts
+    assert counts[1] > 0 and counts[2] > 0
+    assert int(res.attempts.max().item()) == res.iterations
+    assert bool((res.attempts >= res.steps).all())
