@@ -49,7 +49,8 @@ extern "C" {
  * BH_MODE_FAST:    FMA contraction, shared reciprocals, Newton x^-0.2.
  * BH_MODE_PRECISE: one correctly rounded IEEE operation per NumPy ufunc of
  *                  the reference, in the reference's order (no contraction),
- *                  libm-class pow; used to show rounding-level parity. */
+ *                  x ** -0.2 correctly rounded; used to show rounding-level
+ *                  parity. */
 #define BH_MODE_FAST 0
 #define BH_MODE_PRECISE 1
 
