@@ -100,7 +100,9 @@ def test_dp5_step(hip, spin):
     np.testing.assert_array_equal(new, new_r)
     np.testing.assert_array_equal(err, err_r)
     np.testing.assert_array_equal(ratio, ratio_r)
-    np.testing.assert_allclose(fac, fac_r, rtol=4e-16)  # libm-class pow: <= 1-2 ulp
+    # x ** -0.2 is correctly rounded on the device; NumPy's power is off by an
+    # ulp in ~5% of calls, so the factors agree to 1-2 ulp
+    np.testing.assert_allclose(fac, fac_r, rtol=4e-16)
     g = integrators.GeodesicRHS(hip.KerrSpacetime(spin=spin), mode="fast")
     new, err, ratio, fac = integrators.dormand_prince_attempt(g, s, h, 1e-8, 1e-10)
     # FMA-contracted stages round differently; the embedded error is a sum
