@@ -46,7 +46,8 @@ extern "C" {
 #define BH_TERMINATED 6
 
 /* Arithmetic mode of a kernel.
- * BH_MODE_FAST:    FMA contraction, shared reciprocals, Newton x^-0.2.
+ * BH_MODE_FAST:    FMA contraction, shared reciprocals, squared error ratio
+ *                  with a Newton step factor.
  * BH_MODE_PRECISE: one correctly rounded IEEE operation per NumPy ufunc of
  *                  the reference, in the reference's order (no contraction),
  *                  x ** -0.2 correctly rounded; used to show rounding-level
