@@ -103,6 +103,15 @@ def test_dp5_step(hip, spin):
     # x ** -0.2 is correctly rounded on the device; NumPy's power is off by an
     # ulp in ~5% of calls, so the factors agree to 1-2 ulp
     np.testing.assert_allclose(fac, fac_r, rtol=4e-16)
+    # against the correctly rounded factor: x ** -0.2 in extended precision
+    # (the double 0.2 is 3602879701896397 / 2^54) rounded once to double,
+    # then 0.9 * that and the clip, as the reference computes them
+    safe = np.maximum(ratio_r, 1e-16)
+    e = np.longdouble(-3602879701896397) / np.longdouble(2.0 ** 54)
+    pw = np.power(safe.astype(np.longdouble), e).astype(np.float64)
+    fac_cr = np.clip(0.9 * pw, 0.2, 5.0)
+    assert np.mean(fac == fac_cr) > 0.99
+    np.testing.assert_allclose(fac, fac_cr, rtol=2.3e-16)
     g = integrators.GeodesicRHS(hip.KerrSpacetime(spin=spin), mode="fast")
     new, err, ratio, fac = integrators.dormand_prince_attempt(g, s, h, 1e-8, 1e-10)
     # FMA-contracted stages round differently; the embedded error is a sum
