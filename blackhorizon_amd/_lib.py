@@ -170,14 +170,15 @@ def check(rc: int, what: str) -> None:
 
 
 def device_ready() -> bool:
-    """True when a HIP device is visible to torch and the library loads."""
-    try:
-        if not torch.cuda.is_available():
-            return False
-        load()
-        return True
-    except Exception:
+    """True when a HIP device is visible to torch and the library loads
+    (an unbuilt, unloadable or mismatched library counts as not ready)."""
+    if not torch.cuda.is_available():
         return False
+    try:
+        load()
+    except (OSError, RuntimeError, AttributeError):
+        return False
+    return True
 
 
 def require_device() -> ctypes.CDLL:
