@@ -26,10 +26,22 @@ pytestmark = pytest.mark.gpu
 
 # a = 0.9 at 17 deg (the headline spin) and a = 0.998 at 80 deg (the longest)
 FRAMES = [4, 7]
+_SPREAD = {}
 
 
+def _spread(frame, job, g, kw):
+    """The reference's own host-to-host spread on the sampled pixels (cached
+    across modes)."""
+    if frame not in _SPREAD:
+        _SPREAD[frame] = reference_spread(job.spin, g[f"f{frame}_state0"],
+                                          g[f"f{frame}_states"],
+                                          escape_radius=job.escape_radius, **kw)
+    return _SPREAD[frame]
+
+
+@pytest.mark.parametrize("mode", ["auto", "precise"])
 @pytest.mark.parametrize("frame", FRAMES)
-def test_full_hd_frame(hip, frame):
+def test_full_hd_frame(hip, frame, mode):
     from blackhorizon_amd.tracer import Tune
 
     g = golden("hd_subsample.npz")
@@ -46,20 +58,19 @@ def test_full_hd_frame(hip, frame):
     kw = dict(max_steps=job.max_steps, rtol=job.rtol, atol=job.atol,
               capture_margin=job.capture_margin)
     order = torch.from_numpy(cam.center_tile_order(8)).cuda()
-    res = hip.trace_rays(st, s0, job.escape_radius, ray_order=order, **kw)
+    res = hip.trace_rays(st, s0, job.escape_radius, ray_order=order, mode=mode, **kw)
 
     # the sampled pixels of the full launch against the reference's traces
     sel = torch.from_numpy(idx).cuda()
-    spread = reference_spread(job.spin, g[f"f{frame}_state0"], g[f"f{frame}_states"],
-                              escape_radius=job.escape_radius, **kw)
+    spread = _spread(frame, job, g, kw)
     assert_trace_parity(res.states[sel].cpu().numpy(), res.status[sel].cpu().numpy(),
                         res.steps[sel].cpu().numpy(), g[f"f{frame}_states"],
                         g[f"f{frame}_status"], g[f"f{frame}_steps"],
                         label=f"full frame {frame} sample", spread=spread)
 
     # the whole frame: bitwise independent of hand-out order and launch shape
-    other = hip.trace_rays(st, s0, job.escape_radius, tune=Tune(64, 2, 16), **kw)
-    again = hip.trace_rays(st, s0, job.escape_radius, ray_order=order, **kw)
+    other = hip.trace_rays(st, s0, job.escape_radius, tune=Tune(64, 2, 16), mode=mode, **kw)
+    again = hip.trace_rays(st, s0, job.escape_radius, ray_order=order, mode=mode, **kw)
     for r2 in (other, again):
         assert torch.equal(r2.status, res.status)
         assert torch.equal(r2.steps, res.steps)
