@@ -1,5 +1,6 @@
 """GPU: whole 1920x1080 frames of the large-trace configuration (2,073,600
-rays per launch), checked through what does not depend on the size:
+rays per launch), in the auto (here fast) and the precise build, checked
+through what does not depend on the size:
 
 * the device-built camera states at the golden file's 96 seeded pixels equal
   the oracle's bit for bit, and those pixels' results inside the full
