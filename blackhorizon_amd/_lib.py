@@ -112,6 +112,9 @@ SIGNATURES = {
     "bh_geometry_f64": [_D, _D, _I32, _P, _I64, _P, _P, _P, _P, _P, _P],
     "bh_geodesic_rhs_f64": [_D, _D, _I32, _P, _I64, _P, _P],
     "bh_dp5_step_f64": [_D, _D, _I32, _D, _D, _P, _P, _I64, _P, _P, _P, _P, _P],
+    "bh_error_ratio_f64": [_I32, _P, _P, _P, _I64, _I32, _D, _D, _P, _P],
+    "bh_step_factor_f64": [_I32, _P, _I64, _P, _P],
+    "bh_checker_colors_u8": [_P, _I64, _I32, _P, _P],
     "bh_trace_tile_f64": [ctypes.POINTER(TraceParams), ctypes.POINTER(TileParams),
                           ctypes.POINTER(TuneParams), _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P,
                           _P, _P, _P, _I64, _P],

@@ -306,6 +306,42 @@ int32_t bh_geodesic_rhs_f64(double mass, double spin, int32_t mode, const double
   return hip_status(e, "bh_geodesic_rhs_f64");
 }
 
+int32_t bh_error_ratio_f64(int32_t mode, const double *state, const double *new_state,
+                           const double *error, int64_t n, int32_t dim, double rtol,
+                           double atol, double *ratio, void *stream) {
+  int32_t rc = check_mode(mode);
+  if (rc) return rc;
+  if (!rays_ok(n)) return fail(BH_EINVAL, "bad n");
+  if (dim < 1 || dim > 128) return fail(BH_EINVAL, "dim must be 1..128");
+  if (n > 0 && (!state || !new_state || !error || !ratio)) return fail(BH_EINVAL, "NULL buffer");
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e =
+      mode == BH_MODE_PRECISE
+          ? precise::launch_error_ratio(state, new_state, error, n, dim, rtol, atol, ratio, s)
+          : fast::launch_error_ratio(state, new_state, error, n, dim, rtol, atol, ratio, s);
+  return hip_status(e, "bh_error_ratio_f64");
+}
+
+int32_t bh_step_factor_f64(int32_t mode, const double *ratio, int64_t n, double *factor,
+                           void *stream) {
+  int32_t rc = check_mode(mode);
+  if (rc) return rc;
+  if (!rays_ok(n)) return fail(BH_EINVAL, "bad n");
+  if (n > 0 && (!ratio || !factor)) return fail(BH_EINVAL, "NULL buffer");
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = mode == BH_MODE_PRECISE ? precise::launch_step_factor(ratio, n, factor, s)
+                                         : fast::launch_step_factor(ratio, n, factor, s);
+  return hip_status(e, "bh_step_factor_f64");
+}
+
+int32_t bh_checker_colors_u8(const double *directions, int64_t n, int32_t cells, uint8_t *rgb,
+                             void *stream) {
+  if (!rays_ok(n) || cells < 1) return fail(BH_EINVAL, "bad n / cells");
+  if (n > 0 && (!directions || !rgb)) return fail(BH_EINVAL, "NULL buffer");
+  return hip_status(precise::launch_checker_colors(directions, n, cells, rgb, (hipStream_t)stream),
+                    "bh_checker_colors_u8");
+}
+
 int32_t bh_dp5_step_f64(double mass, double spin, int32_t mode, double rtol, double atol,
                         const double *state, const double *h, int64_t n, double *new_state,
                         double *error, double *ratio, double *factor, void *stream) {

@@ -209,6 +209,14 @@ struct LaunchShape {
                              double *gl, hipStream_t s);                           \
   hipError_t launch_rhs(const JobConst &c, const double *state, int64_t n,         \
                         double *deriv, hipStream_t s);                             \
+  hipError_t launch_error_ratio(const double *state, const double *new_state,     \
+                                const double *error, int64_t n, int dim,           \
+                                double rtol, double atol, double *ratio,           \
+                                hipStream_t s);                                    \
+  hipError_t launch_step_factor(const double *ratio, int64_t n, double *factor,    \
+                                hipStream_t s);                                    \
+  hipError_t launch_checker_colors(const double *dirs, int64_t n, int cells,       \
+                                   uint8_t *rgb, hipStream_t s);                   \
   hipError_t launch_dp5_step(const JobConst &c, const double *state,               \
                              const double *h, int64_t n, double *new_state,        \
                              double *err, double *ratio, double *factor,           \

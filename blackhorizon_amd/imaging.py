@@ -23,6 +23,18 @@ COLOR_MAX_STEPS = (120, 32, 32)
 COLOR_FAILED = (200, 0, 200)
 
 
+def celestial_checkerboard(directions, cells: int = 12) -> numpy.ndarray:
+    """RGB uint8 (n, 3) checker colours of unit directions (n, 3) on the
+    celestial sphere, ``cells`` cells over the polar angle (imaging.py:27-50)."""
+    d = _dev.as_device_f64(directions, (3,), "directions")
+    n = d.shape[0]
+    rgb = torch.empty((n, 3), dtype=torch.uint8, device=d.device)
+    lib = _lib.require_device()
+    _lib.check(lib.bh_checker_colors_u8(_lib.ptr(d), n, int(cells), _lib.ptr(rgb),
+                                        _lib.stream_handle()), "celestial_checkerboard")
+    return to_numpy(rgb)
+
+
 def render_image(spacetime: KerrSpacetime, result: TraceResult, width: int, height: int,
                  cells: int = 12) -> numpy.ndarray:
     """RGB uint8 image (height, width, 3) of ``height * width`` traced rays

@@ -83,6 +83,46 @@ def dormand_prince_attempt(rhs, state: Array, h: Array, rtol: float, atol: float
     return tuple(_dev.like_input(t, state) for t in out)
 
 
+def error_ratio(state: Array, new_state: Array, error: Array, rtol: float, atol: float, *,
+                mode: str = "precise") -> Array:
+    """Scaled RMS error over the last axis (integrators.py:110-117): a value
+    <= 1 accepts the step; non-finite values become +inf.  The three arrays
+    share one shape (..., d) with 1 <= d <= 128; the result has shape (...).
+    The precise build matches NumPy's rounding and summation order bit for
+    bit."""
+    s = _dev.as_device_f64(state)
+    sn = _dev.as_device_f64(new_state)
+    se = _dev.as_device_f64(error)
+    if s.dim() == 0 or sn.shape != s.shape or se.shape != s.shape:
+        raise ValueError("state, new_state and error must share one shape (..., d)")
+    d = s.shape[-1]
+    if not 1 <= d <= 128:
+        raise ValueError("the last axis must hold 1 to 128 values")
+    lead = s.shape[:-1]
+    n = s.numel() // d
+    ratio = torch.empty(n, dtype=torch.float64, device=s.device)
+    lib = _lib.require_device()
+    _lib.check(lib.bh_error_ratio_f64(_mode(mode), _lib.ptr(s), _lib.ptr(sn), _lib.ptr(se), n, d,
+                                      float(rtol), float(atol), _lib.ptr(ratio),
+                                      _lib.stream_handle()), "error_ratio")
+    return _dev.like_input(ratio.reshape(lead), state)
+
+
+def step_factor(ratio: Array, *, mode: str = "precise") -> Array:
+    """Step-size multiplier clip(0.9 max(ratio, 1e-16)^-0.2, 0.2, 5)
+    (integrators.py:120-124); the precise build rounds the power correctly."""
+    is_t = isinstance(ratio, torch.Tensor)
+    r = ratio.detach().to(_dev.device(), torch.float64).contiguous() if is_t else \
+        _dev.as_device_f64(ratio)
+    shape = r.shape
+    r = r.reshape(-1)
+    out = torch.empty_like(r)
+    lib = _lib.require_device()
+    _lib.check(lib.bh_step_factor_f64(_mode(mode), _lib.ptr(r), r.shape[0], _lib.ptr(out),
+                                      _lib.stream_handle()), "step_factor")
+    return _dev.like_input(out.reshape(shape), ratio)
+
+
 def _fixed_steps(rhs, state: Array, h: Array, n_steps: int, mode, launch, what: str) -> Array:
     g = _resolve(rhs)
     s = _dev.as_device_f64(state, (8,), "state")

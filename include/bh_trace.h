@@ -219,6 +219,21 @@ int32_t bh_geodesic_rhs_f64(double mass, double spin, int32_t mode,
                             const double *state, int64_t n, double *deriv,
                             void *stream);
 
+/* error_ratio (integrators.py:110-117) of n rows of dim (1..128) values of
+ * state, new_state and error, each (n, dim) row-major: ratio (n) =
+ * sqrt(mean((error / (atol + rtol max(|state|, |new_state|)))^2)),
+ * non-finite -> +inf.  Precise: NumPy's operations and summation order, bit
+ * for bit. */
+int32_t bh_error_ratio_f64(int32_t mode, const double *state, const double *new_state,
+                           const double *error, int64_t n, int32_t dim, double rtol,
+                           double atol, double *ratio, void *stream);
+
+/* step_factor (integrators.py:120-124) of n ratios: clip(0.9 max(ratio,
+ * 1e-16)^-0.2, 0.2, 5).  Precise: the power correctly rounded; fast: the
+ * fast build's Newton power of the squared ratio. */
+int32_t bh_step_factor_f64(int32_t mode, const double *ratio, int64_t n, double *factor,
+                           void *stream);
+
 /* dormand_prince_step (integrators.py:76-107) with the geodesic RHS, then
  * error_ratio (110-117) and step_factor (120-124): (n, 8) rows and h (n) ->
  * new_state (n, 8), error (n, 8), ratio (n), factor (n).  ratio/factor may
@@ -344,6 +359,11 @@ int32_t bh_integrate_dense_f64(double mass, double spin, int32_t mode, double rt
 int32_t bh_camera_rays_f64(double mass, double spin, const double *camera, double half_w,
                            double half_h, int64_t width, int64_t height, int32_t past,
                            double *state_soa, int64_t ld, void *stream);
+
+/* celestial_checkerboard (imaging.py:27-50): RGB uint8 (n, 3) of n unit
+ * directions (n, 3) on `cells` polar cells (cells >= 1). */
+int32_t bh_checker_colors_u8(const double *directions, int64_t n, int32_t cells, uint8_t *rgb,
+                             void *stream);
 
 /* render_image (imaging.py:53-91) on the device: RGB uint8 (n, 3) per ray
  * from final states (SoA) and status -- captured (0,0,0), escaped by the

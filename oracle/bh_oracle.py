@@ -584,6 +584,20 @@ def camera_state(st: Kerr, cam: PinholeCamera):
     return build_state(pos, mom)
 
 
+def celestial_checkerboard(directions, cells=12):
+    """imaging.py:27-50: light (104,128,178) where the polar and azimuthal
+    cell indices have odd sum, dark (24,30,52) elsewhere."""
+    z = np.clip(directions[:, 2], -1.0, 1.0)
+    theta = np.arccos(z)
+    phi = np.arctan2(directions[:, 1], directions[:, 0])
+    i_theta = np.floor(theta / np.pi * cells).astype(np.int64)
+    i_phi = np.floor((phi + np.pi) / (2.0 * np.pi) * 2 * cells).astype(np.int64)
+    parity = ((i_theta + i_phi) % 2).astype(bool)
+    light = np.array([104, 128, 178], dtype=np.uint8)
+    dark = np.array([24, 30, 52], dtype=np.uint8)
+    return np.where(parity[:, None], light, dark).astype(np.uint8)
+
+
 def checker_parity(st: Kerr, states):
     """Light (True) / dark (False) cell of the escape direction; imaging.py:27-50, 77-81."""
     v = coordinate_velocity(st, states)[:, 1:4]

@@ -410,6 +410,35 @@ std::vector<Case> build_cases() {
   add("shade: rgb NULL", [=] {
     return bh_shade_checker_u8(1.0, 0.5, d, 4, i32, 4, 8, nullptr, nullptr);
   }, "NULL buffer");
+  add("checker colours: cells -3", [=] { return bh_checker_colors_u8(d, 4, -3, g_b, nullptr); },
+      "bad n / cells");
+  add("checker colours: n INT64_MAX", [=] {
+    return bh_checker_colors_u8(d, BIG, 12, g_b, nullptr);
+  }, "bad n / cells");
+  add("checker colours: directions NULL", [=] {
+    return bh_checker_colors_u8(nullptr, 4, 12, g_b, nullptr);
+  }, "NULL buffer");
+
+  // --- controller helpers ---
+  add("error ratio: mode 9", [=] {
+    return bh_error_ratio_f64(9, d, d, d, 4, 8, 1e-8, 1e-10, d, nullptr);
+  }, "mode must be");
+  add("error ratio: n INT64_MAX", [=] {
+    return bh_error_ratio_f64(BH_MODE_PRECISE, d, d, d, BIG, 8, 1e-8, 1e-10, d, nullptr);
+  }, "bad n");
+  add("error ratio: dim 129", [=] {
+    return bh_error_ratio_f64(BH_MODE_FAST, d, d, d, 4, 129, 1e-8, 1e-10, d, nullptr);
+  }, "dim must be");
+  add("error ratio: ratio NULL", [=] {
+    return bh_error_ratio_f64(BH_MODE_PRECISE, d, d, d, 4, 2, 1e-8, 1e-10, nullptr, nullptr);
+  }, "NULL buffer");
+  add("step factor: mode -1", [=] { return bh_step_factor_f64(-1, d, 4, d, nullptr); },
+      "mode must be");
+  add("step factor: n -4", [=] { return bh_step_factor_f64(BH_MODE_FAST, d, -4, d, nullptr); },
+      "bad n");
+  add("step factor: factor NULL", [=] {
+    return bh_step_factor_f64(BH_MODE_PRECISE, d, 4, nullptr, nullptr);
+  }, "NULL buffer");
 
   // --- momenta and diagnostics ---
   add("null momentum: n -1", [=] {

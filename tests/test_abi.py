@@ -107,3 +107,30 @@ def test_new_entries_reject_bad_arguments():
     assert lib.bh_kernel_attributes(0, 0, 96, ctypes.addressof(out)) == E
     assert lib.bh_adjoint_scratch_doubles(0) == -1
     assert lib.bh_adjoint_scratch_doubles(32) == 9 * 32
+
+
+def test_controller_and_colour_entries_reject_bad_arguments():
+    """error_ratio / step_factor / checker colours refuse unknown modes, bad
+    counts, missing buffers and a zero cell count before any device work."""
+    lib = _lib.load()
+    E = _lib.BH_EINVAL
+    buf = (ctypes.c_double * 8)()
+    p = ctypes.addressof(buf)
+    assert lib.bh_error_ratio_f64(7, p, p, p, 1, 8, 1e-8, 1e-10, p, None) == E
+    assert "mode" in lib.bh_last_error().decode()
+    assert lib.bh_error_ratio_f64(0, p, p, p, -1, 8, 1e-8, 1e-10, p, None) == E
+    assert lib.bh_error_ratio_f64(0, p, p, p, 1, 0, 1e-8, 1e-10, p, None) == E
+    assert lib.bh_error_ratio_f64(1, p, p, p, 1, 129, 1e-8, 1e-10, p, None) == E
+    assert "dim" in lib.bh_last_error().decode()
+    assert lib.bh_error_ratio_f64(0, p, None, p, 1, 8, 1e-8, 1e-10, p, None) == E
+    assert lib.bh_step_factor_f64(3, p, 1, p, None) == E
+    assert lib.bh_step_factor_f64(0, p, 1 << 40, p, None) == E
+    assert lib.bh_step_factor_f64(1, None, 1, p, None) == E
+    rgb = (ctypes.c_uint8 * 3)()
+    assert lib.bh_checker_colors_u8(p, 1, 0, ctypes.addressof(rgb), None) == E
+    assert lib.bh_checker_colors_u8(p, -2, 12, ctypes.addressof(rgb), None) == E
+    assert lib.bh_checker_colors_u8(p, 1, 12, None, None) == E
+    # n = 0 is a no-op that succeeds without a device
+    assert lib.bh_error_ratio_f64(0, None, None, None, 0, 2, 1e-8, 1e-10, None, None) == 0
+    assert lib.bh_step_factor_f64(1, None, 0, None, None) == 0
+    assert lib.bh_checker_colors_u8(None, 0, 12, None, None) == 0

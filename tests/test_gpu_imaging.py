@@ -74,3 +74,27 @@ def test_shadow_gpu_png_rendered_on_device(hip):
     esc = cls != 0
     assert np.all(got[esc] != 3)
     assert np.mean(got[esc] == cls[esc]) > 0.999
+
+
+def test_celestial_checkerboard_matches_oracle(hip):
+    """celestial_checkerboard (imaging.py:27-50) on the device equals the
+    oracle on 20000 random unit directions, the cell centres and the poles
+    and axes (whose cell indices fall on boundaries), for several cell
+    counts."""
+    from blackhorizon_amd import imaging
+    from test_oracle_imaging import cell_centres
+
+    rng = np.random.default_rng(11)
+    d = rng.normal(size=(20000, 3))
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    axes = np.array([[0, 0, 1], [0, 0, -1], [1, 0, 0], [-1, 0, 0], [-1, -0.0, 0], [0, 1, 0],
+                     [0, -1, 0], [0, 0, 7.5], [0, 0, -3.0]], dtype=np.float64)
+    for cells in (1, 5, 12, 31):
+        dirs = np.concatenate([d, cell_centres(cells)[0], axes])
+        got = imaging.celestial_checkerboard(dirs, cells)
+        assert got.dtype == np.uint8 and got.shape == dirs.shape
+        np.testing.assert_array_equal(got, orc.celestial_checkerboard(dirs, cells))
+    t = torch.from_numpy(d).cuda()
+    np.testing.assert_array_equal(imaging.celestial_checkerboard(t),
+                                  orc.celestial_checkerboard(d))
+    assert imaging.celestial_checkerboard(np.zeros((0, 3))).shape == (0, 3)

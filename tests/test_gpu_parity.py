@@ -122,6 +122,97 @@ def test_dp5_step(hip, spin):
     np.testing.assert_allclose(fac, orc.step_factor(ratio), rtol=3e-15)
 
 
+def _controller_rows(n, dim, seed):
+    """Rows of dim values over ~17 decades of |y| and ~17 of |err|, with
+    rows that hold a NaN, an infinity, a zero scale and an all-zero error."""
+    rng = np.random.default_rng(seed)
+    y = rng.normal(size=(n, dim)) * np.exp(rng.uniform(-20.0, 20.0, (n, dim)))
+    yn = y + 1e-3 * np.abs(y) * rng.normal(size=(n, dim))
+    err = rng.normal(size=(n, dim)) * np.exp(rng.uniform(-40.0, 0.0, (n, dim)))
+    err[0, 0] = np.nan
+    err[1, dim - 1] = np.inf
+    y[2, :] = yn[2, :] = 0.0  # scale = atol = 0 below: 0/0 and x/0
+    err[3, :] = 0.0
+    return y, yn, err
+
+
+@pytest.mark.parametrize("dim", [1, 2, 7, 8, 9, 16, 23, 128])
+def test_error_ratio_any_row_width(hip, dim):
+    """error_ratio (integrators.py:110-117) over rows of 1..128 values: the
+    precise build equals NumPy bit for bit (division, squares, NumPy's
+    summation order for that width, mean); non-finite rows give +inf; the
+    fast build agrees to 1e-14 relative."""
+    import torch
+
+    from blackhorizon_amd import integrators
+
+    y, yn, err = _controller_rows(1000, dim, 100 + dim)
+    for atol in (1e-12, 0.0):
+        want = orc.error_ratio(y, yn, err, 1e-9, atol)
+        got = integrators.error_ratio(y, yn, err, 1e-9, atol)
+        np.testing.assert_array_equal(got, want)
+        fast = integrators.error_ratio(y, yn, err, 1e-9, atol, mode="fast")
+        np.testing.assert_array_equal(np.isinf(fast), np.isinf(want))
+        np.testing.assert_allclose(fast, want, rtol=1e-14)
+    assert np.isinf(want[:3]).all() and want[3] == 0.0
+    # leading axes and tensors: (4, 250, dim) -> (4, 250); one row -> 0-d
+    t = [torch.from_numpy(a.reshape(4, 250, dim)).cuda() for a in (y, yn, err)]
+    got_t = integrators.error_ratio(*t, 1e-9, 1e-12)
+    assert isinstance(got_t, torch.Tensor) and got_t.shape == (4, 250)
+    np.testing.assert_array_equal(got_t.cpu().numpy().reshape(-1),
+                                  orc.error_ratio(y, yn, err, 1e-9, 1e-12))
+    one = integrators.error_ratio(y[5], yn[5], err[5], 1e-9, 1e-12)
+    assert np.shape(one) == ()
+    assert one == orc.error_ratio(y[5:6], yn[5:6], err[5:6], 1e-9, 1e-12)[0]
+
+
+def test_error_ratio_reference_nan_case(hip):
+    """The reference's own case (tests/test_integrators.py:71-75): a NaN in
+    a two-value row gives an infinite ratio."""
+    from blackhorizon_amd import integrators
+
+    state = np.array([[1.0, 0.0]])
+    ratio = integrators.error_ratio(state, state, np.array([[np.nan, 0.0]]), rtol=1e-9,
+                                    atol=1e-12)
+    assert np.isinf(ratio[0])
+
+
+def test_step_factor_api(hip):
+    """step_factor (integrators.py:120-124) over 100 decades of ratios and the
+    special values: the precise build's power is correctly rounded (against
+    x ** -0.2 in extended precision, rounded once), so it equals that
+    reference in > 99% of values and is within an ulp everywhere; NaN stays
+    NaN, 0, negative and tiny ratios give 5, +inf gives 0.2.  The fast build
+    is within 3e-15 relative."""
+    import torch
+
+    from blackhorizon_amd import integrators
+
+    rng = np.random.default_rng(21)
+    special = np.array([0.0, -1.0, -np.inf, 1e-16, 1e-17, 1e-300, np.inf, np.nan, 1.0,
+                        32.0 ** 5 / 1e5, 1e6, 1e-6, 0.9 ** 5 * 3125.0, 5e-324])
+    ratio = np.concatenate([np.exp(rng.uniform(-115.0, 115.0, 20000)), special])
+    want = orc.step_factor(ratio)
+    safe = np.maximum(ratio, 1e-16)
+    e = np.longdouble(-3602879701896397) / np.longdouble(2.0 ** 54)
+    with np.errstate(invalid="ignore"):
+        pw = np.power(safe.astype(np.longdouble), e).astype(np.float64)
+    want_cr = np.clip(0.9 * pw, 0.2, 5.0)
+    got = integrators.step_factor(ratio)
+    np.testing.assert_array_equal(np.isnan(got), np.isnan(ratio))
+    ok = ~np.isnan(ratio)
+    assert np.mean(got[ok] == want_cr[ok]) > 0.99
+    np.testing.assert_allclose(got[ok], want_cr[ok], rtol=2.3e-16)
+    np.testing.assert_allclose(got, want, rtol=4e-16)
+    np.testing.assert_array_equal(got[-14:-7], [5.0, 5.0, 5.0, 5.0, 5.0, 5.0, 0.2])
+    fast = integrators.step_factor(ratio, mode="fast")
+    np.testing.assert_allclose(fast, want, rtol=3e-15)
+    t = torch.from_numpy(ratio[:20000].reshape(100, 200)).cuda()
+    got_t = integrators.step_factor(t)
+    assert isinstance(got_t, torch.Tensor) and got_t.shape == (100, 200)
+    np.testing.assert_array_equal(got_t.cpu().numpy().reshape(-1), got[:20000])
+
+
 @pytest.mark.parametrize("spin", SPINS)
 def test_initial_conditions_and_diagnostics(hip, spin):
     from blackhorizon_amd import geodesics

@@ -92,3 +92,40 @@ def test_error_ratio_and_step_factor_known_answers():
     np.testing.assert_allclose(r, [np.sqrt(0.625)], rtol=1e-15)
     f = orc.step_factor(np.array([1.0, 1e-30, np.inf, 32.0 ** 5 / 1e5]))
     np.testing.assert_allclose(f, [0.9, 5.0, 0.2, 0.9 * (32.0 ** 5 / 1e5) ** -0.2], rtol=1e-15)
+
+
+def _row_sum(values):
+    """The device kernels' row sum (error_ratio_kernel): sequential from 0
+    below 8 values, else 8 interleaved partial sums joined pairwise, then the
+    remainder sequentially."""
+    d = len(values)
+    if d < 8:
+        s = 0.0
+        for v in values:
+            s += v
+        return s
+    r = [float(v) for v in values[:8]]
+    k = 8
+    while k < d - d % 8:
+        for j in range(8):
+            r[j] += values[k + j]
+        k += 8
+    s = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]))
+    for v in values[k:]:
+        s += v
+    return s
+
+
+def test_row_sum_order_is_numpys():
+    """NumPy's mean over the last axis (the reference's error_ratio,
+    integrators.py:116) sums each row of up to 128 values in the order the
+    device kernels use, bit for bit; at 129 values it splits differently,
+    which is why the device entry takes at most 128."""
+    rng = np.random.default_rng(3)
+    for d in list(range(1, 34)) + [63, 64, 65, 100, 127, 128]:
+        x = rng.random((64, d)) * np.exp(rng.normal(0.0, 5.0, (64, d)))
+        mine = np.array([_row_sum([float(v) for v in row]) for row in x]) / float(d)
+        np.testing.assert_array_equal(mine, np.mean(x, axis=-1), err_msg=f"d={d}")
+    x = rng.random((64, 129)) * np.exp(rng.normal(0.0, 5.0, (64, 129)))
+    mine = np.array([_row_sum([float(v) for v in row]) for row in x]) / 129.0
+    assert (mine != np.mean(x, axis=-1)).any()
