@@ -47,7 +47,7 @@ extern "C" {
 
 /* Arithmetic mode of a kernel.
  * BH_MODE_FAST:    FMA contraction, shared reciprocals, squared error ratio
- *                  with a Newton step factor.
+ *                  with a polynomial-corrected step factor.
  * BH_MODE_PRECISE: one correctly rounded IEEE operation per NumPy ufunc of
  *                  the reference, in the reference's order (no contraction),
  *                  x ** -0.2 correctly rounded; used to show rounding-level
@@ -230,7 +230,8 @@ int32_t bh_error_ratio_f64(int32_t mode, const double *state, const double *new_
 
 /* step_factor (integrators.py:120-124) of n ratios: clip(0.9 max(ratio,
  * 1e-16)^-0.2, 0.2, 5).  Precise: the power correctly rounded; fast: the
- * fast build's Newton power of the squared ratio. */
+ * fast build's power of the squared ratio (float estimate, one second-order
+ * correction). */
 int32_t bh_step_factor_f64(int32_t mode, const double *ratio, int64_t n, double *factor,
                            void *stream);
 
