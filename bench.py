@@ -219,7 +219,9 @@ def run_uneven(args) -> dict:
         tjobs.append(batch.TraceJob(st, state, job.escape_radius, job.max_steps, job.rtol,
                                     job.atol, job.initial_step, job.max_step, job.min_step,
                                     job.capture_margin))
-    plan = batch.BatchPlan(tjobs, mode=args.mode, concurrent=not args.serial_groups)
+    plan = batch.BatchPlan(tjobs, mode=args.mode, concurrent=not args.serial_groups,
+                           order_fn=batch.longest_first if args.batch_order == "longest_first"
+                           else None)
     prepared = prepare_gpu(jobs, args.mode, "row")
     for t in tjobs:
         t.state0 = None
@@ -284,7 +286,8 @@ def run_uneven(args) -> dict:
         ray_steps=steps_b, attempts=att_b, rays=plan.total_rays,
         pct_fp64_vector_peak_batched=100.0 * att_b * FLOP_PER_ATTEMPT / (ms_batched * 1e-3)
         / (FP64_VECTOR_PEAK_TFLOPS * 1e12),
-        jobs_per_mode=modes, per_ray_steps_identical=bool(same and steps_b == steps_p))
+        jobs_per_mode=modes, queue_order=args.batch_order,
+        per_ray_steps_identical=bool(same and steps_b == steps_p))
 
 
 RK4_FLOP_PER_STEP = 4 * 146 + 104  # 4 RHS + stage and update arithmetic, 8 components
@@ -633,6 +636,8 @@ def main(argv=None) -> int:
     ap.add_argument("--fit-size", type=int, default=512, help="shadowfit: image side")
     ap.add_argument("--offline-size", type=int, default=1920,
                     help="offline: frame width (height = 5/8 of it)")
+    ap.add_argument("--batch-order", default="longest_first", choices=["longest_first", "given"],
+                    help="uneven: queue order of the batched launch (per-job: row order)")
     ap.add_argument("--serial-groups", action="store_true",
                     help="uneven: run the fast and precise batches one after the other")
     ap.add_argument("--mode", default="auto", choices=["auto", "fast", "precise"])

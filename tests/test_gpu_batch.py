@@ -117,10 +117,11 @@ def test_batch_order_does_not_change_results(hip):
     def reverse(jobs_in_group, offsets):
         return np.arange(int(offsets[-1]))[::-1].copy()
 
-    a = batch.trace_batch(jobs, mode="precise")
-    b = batch.trace_batch(jobs, mode="precise", order_fn=reverse)
-    for r, r2 in zip(a, b):
-        _assert_same(r, r2, "reversed queue")
+    a = batch.trace_batch(jobs, mode="precise")  # longest_first
+    for fn, label in ((reverse, "reversed queue"), (None, "given order")):
+        b = batch.trace_batch(jobs, mode="precise", order_fn=fn)
+        for r, r2 in zip(a, b):
+            _assert_same(r, r2, label)
 
 
 @pytest.mark.parametrize("tile", [4, 8, 16])
