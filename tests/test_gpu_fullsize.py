@@ -1,6 +1,6 @@
 """GPU: whole 1920x1080 frames of the large-trace configuration (2,073,600
-rays per launch), all eight in the auto (here fast) build and two in the
-precise build, checked through what does not depend on the size:
+rays per launch), all eight in the auto (here fast) and in the precise
+build, checked through what does not depend on the size:
 
 * the device-built camera states at the golden file's 96 seeded pixels equal
   the oracle's bit for bit, and those pixels' results inside the full
@@ -25,9 +25,8 @@ from blackhorizon_amd import workloads
 
 pytestmark = pytest.mark.gpu
 
-# every frame of the sweep in the default mode; the precise build on a = 0.9
-# at 17 deg (the headline spin) and a = 0.998 at 80 deg (the longest)
-CASES = [(f, "auto") for f in range(8)] + [(4, "precise"), (7, "precise")]
+# every frame of the sweep in both builds
+CASES = [(f, m) for m in ("auto", "precise") for f in range(8)]
 _SPREAD = {}
 
 
