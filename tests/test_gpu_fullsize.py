@@ -1,6 +1,7 @@
-"""GPU: whole 1920x1080 frames of the large-trace configuration (2,073,600
-rays per launch), all eight in the auto (here fast) and in the precise
-build, checked through what does not depend on the size:
+"""GPU: the full-size workloads.  Whole 1920x1080 frames of the
+large-trace configuration (2,073,600 rays per launch), all eight in the
+auto (here fast) and in the precise build, checked through what does not
+depend on the size:
 
 * the device-built camera states at the golden file's 96 seeded pixels equal
   the oracle's bit for bit, and those pixels' results inside the full
@@ -13,6 +14,8 @@ build, checked through what does not depend on the size:
 * p_t is carried exactly (E conserved bit for bit), every ray ends
   captured or escaped, and the launch's iteration count is the largest
   per-ray attempt count.
+
+The whole 64-job uneven mix is checked the same way (test_full_uneven_mix).
 """
 
 import numpy as np
@@ -85,3 +88,49 @@ def test_full_hd_frame(hip, frame, mode):
     assert counts[1] > 0 and counts[2] > 0
     assert int(res.attempts.max().item()) == res.iterations
     assert bool((res.attempts >= res.steps).all())
+
+
+def test_full_uneven_mix(hip):
+    """The whole 64-job uneven mix (2.19 M rays, spins of both signs,
+    r_obs 20-1000 M, rtol 1e-10..1e-6, both builds) in one batched plan
+    with its default longest-first queue: every job's results equal a plain
+    trace_rays launch of that job bit for bit, the golden-sampled rays of
+    the first eight jobs meet the parity bar inside the full launch, and
+    p_t is carried exactly."""
+    from blackhorizon_amd import batch
+
+    g = golden("uneven_subsample.npz")
+    mix = workloads.uneven_mix()
+    jobs, spaces = [], []
+    for j in mix:
+        st = hip.KerrSpacetime(spin=j.spin)
+        cam = hip.PinholeCamera.from_orbit(j.distance, j.inclination_degrees,
+                                           fov_degrees=j.fov_degrees, width=j.width,
+                                           height=j.height)
+        spaces.append(st)
+        jobs.append(batch.TraceJob(st, cam.device_states(st), j.escape_radius, j.max_steps,
+                                   j.rtol, j.atol, j.initial_step, j.max_step, j.min_step,
+                                   j.capture_margin))
+    plan = batch.BatchPlan(jobs, mode="auto")
+    assert len(plan.groups) == 2
+    plan.launch()
+    res = plan.results()
+    for k, (j, tj, r) in enumerate(zip(mix, jobs, res)):
+        alone = hip.trace_rays(spaces[k], tj.state0, j.escape_radius, j.max_steps, j.rtol,
+                               j.atol, j.initial_step, j.max_step, j.min_step, j.capture_margin)
+        assert torch.equal(r.status, alone.status), k
+        assert torch.equal(r.steps, alone.steps), k
+        assert torch.equal(r.attempts, alone.attempts), k
+        assert torch.equal(r.states.view(torch.int64), alone.states.view(torch.int64)), k
+        assert r.iterations == alone.iterations, k
+        assert torch.equal(r.states[:, 4].view(torch.int64), tj.state0[:, 4].view(torch.int64))
+        if k < 8:
+            idx = torch.from_numpy(g[f"j{k}_index"]).cuda()
+            np.testing.assert_array_equal(tj.state0[idx].cpu().numpy(), g[f"j{k}_state0"])
+            kw = dict(escape_radius=j.escape_radius, max_steps=j.max_steps, rtol=j.rtol,
+                      atol=j.atol, capture_margin=j.capture_margin)
+            spread = reference_spread(j.spin, g[f"j{k}_state0"], g[f"j{k}_states"], **kw)
+            assert_trace_parity(r.states[idx].cpu().numpy(), r.status[idx].cpu().numpy(),
+                                r.steps[idx].cpu().numpy(), g[f"j{k}_states"],
+                                g[f"j{k}_status"], g[f"j{k}_steps"],
+                                label=f"uneven job {k} in the full launch", spread=spread)
