@@ -538,7 +538,8 @@ def run_shadowfit(args) -> dict:
                 ms_adjoint=med["adjoint"], ms_fit_step=med["fit_step"],
                 adjoint_over_forward=med["adjoint"] / med["plain"],
                 forward_plus_adjoint_over_forward=(med["record"] + med["adjoint"]) / med["plain"],
-                status_counts=torch.bincount(status.long(), minlength=7).tolist())
+                status_counts=torch.bincount(status.long(), minlength=7).tolist(),
+                peak_device_gb=torch.cuda.max_memory_allocated() / 1e9)
 
 
 def run_orbits(args) -> dict:
