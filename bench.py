@@ -648,7 +648,7 @@ def main(argv=None) -> int:
     ap.add_argument("--order", default="center8",
                     help="ray hand-out order: row (pixel order), tile<k> (k x k pixel tiles, "
                          "row-major) or center<k> (k x k tiles, nearest the image centre first)")
-    ap.add_argument("--cpu-rays-per-frame", type=int, default=6144)
+    ap.add_argument("--cpu-rays-per-frame", type=int, default=12288)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="hd/hd09: skip the adjoint/forward measurement (512^2 shadow fit)")
