@@ -90,9 +90,11 @@ def test_full_hd_frame(hip, frame, mode):
     assert bool((res.attempts >= res.steps).all())
 
 
-def test_full_uneven_mix(hip):
+@pytest.mark.parametrize("mode", ["auto", "precise"])
+def test_full_uneven_mix(hip, mode):
     """The whole 64-job uneven mix (2.19 M rays, spins of both signs,
-    r_obs 20-1000 M, rtol 1e-10..1e-6, both builds) in one batched plan
+    r_obs 20-1000 M, rtol 1e-10..1e-6; auto splits it between the two builds,
+    precise runs every job bit-exact) in one batched plan
     with its default longest-first queue: every job's results equal a plain
     trace_rays launch of that job bit for bit, the golden-sampled rays of
     the first eight jobs meet the parity bar inside the full launch, and
@@ -111,13 +113,14 @@ def test_full_uneven_mix(hip):
         jobs.append(batch.TraceJob(st, cam.device_states(st), j.escape_radius, j.max_steps,
                                    j.rtol, j.atol, j.initial_step, j.max_step, j.min_step,
                                    j.capture_margin))
-    plan = batch.BatchPlan(jobs, mode="auto")
-    assert len(plan.groups) == 2
+    plan = batch.BatchPlan(jobs, mode=mode)
+    assert len(plan.groups) == (2 if mode == "auto" else 1)
     plan.launch()
     res = plan.results()
     for k, (j, tj, r) in enumerate(zip(mix, jobs, res)):
         alone = hip.trace_rays(spaces[k], tj.state0, j.escape_radius, j.max_steps, j.rtol,
-                               j.atol, j.initial_step, j.max_step, j.min_step, j.capture_margin)
+                               j.atol, j.initial_step, j.max_step, j.min_step, j.capture_margin,
+                               mode=mode)
         assert torch.equal(r.status, alone.status), k
         assert torch.equal(r.steps, alone.steps), k
         assert torch.equal(r.attempts, alone.attempts), k
