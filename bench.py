@@ -42,7 +42,7 @@ FP64_VECTOR_PEAK_TFLOPS = 78.6
 # HBM bytes per trace_plain_kernel launch on the 1920x1080 frames (fast
 # build): rocprofv3 FETCH_SIZE + WRITE_SIZE, each calibrated on the layout
 # copy aos_to_soa_kernel of the same run (profiles/trace_plain_hd_sweep.md).
-HD_TRAFFIC_BYTES_PER_LAUNCH = 3.985e8
+HD_TRAFFIC_BYTES_PER_LAUNCH = 3.982e8
 METRIC = "ray-steps/s FP64 @rtol 1e-8, % FP64 vector peak, adjoint/forward time"
 
 
