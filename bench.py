@@ -542,6 +542,55 @@ def run_shadowfit(args) -> dict:
                 peak_device_gb=torch.cuda.max_memory_allocated() / 1e9)
 
 
+def run_shader(args) -> dict:
+    """The real-time shader's fixed-step algorithm (realtime/reference.py:
+    41-188) over a 1920x1080 frame per quality preset, a = 0.9, camera at
+    30 M and inclination 80 deg, FOV 60 deg, equatorial disk from the ISCO to
+    18 M, the bit-exact build; plus the image stage at the same size: camera
+    states built on the device and the Stage-1 shading of a traced frame.
+    HIP-event times over the timed passes (rays resident on the device)."""
+    import torch
+
+    from blackhorizon_amd import _lib, imaging, kerr, offline, realtime, tracer
+    from blackhorizon_amd.camera import PinholeCamera
+
+    _lib.require_device()
+    w, h = 1920, 1080
+    st = kerr.KerrSpacetime(1.0, 0.9)
+    cam = PinholeCamera.from_orbit(30.0, 80.0, fov_degrees=60.0, width=w, height=h)
+    s0 = cam.device_states(st)
+    disk = (offline.disk_inner_radius(st), 18.0)
+    cur = torch.cuda.current_stream()
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(cur)
+        for _ in range(args.steps):
+            out = fn()
+        e1.record(cur)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.steps, out
+
+    presets = {}
+    for preset in realtime.QualityPreset:
+        cfg = realtime.RenderSettings().apply_preset(preset)
+        esc = cfg.effective_escape_radius(30.0)
+        ms, res = timed(lambda: realtime.trace_like_shader(st, s0, cfg, esc, disk))
+        counts = torch.bincount(res.status.long(), minlength=7).tolist()
+        presets[preset.value] = dict(max_steps=cfg.max_steps, step_scale=cfg.step_scale,
+                                     ms=ms, rays_per_s=w * h / (ms * 1e-3),
+                                     status_counts=counts)
+        print(f"[bench] shader preset {preset.value}: {ms:.2f} ms", file=sys.stderr, flush=True)
+    ms_cam, _ = timed(lambda: cam.device_states(st))
+    traced = tracer.trace_rays(st, s0, 60.0, rtol=1e-8, atol=1e-10)
+    ms_shade, _ = timed(lambda: imaging.render_image(st, traced, w, h))
+    return dict(rays=w * h, presets=presets, ms_camera_states=ms_cam,
+                ms_render_image_incl_host_copy=ms_shade)
+
+
 def run_orbits(args) -> dict:
     """65,536 near-circular timelike orbits around a = 0.9 in 16 uneven jobs,
     integrated to proper time 1e4 M at rtol 1e-10 in one persistent batched
@@ -631,7 +680,7 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="hd",
                     choices=["hd", "hd09", "small", "uneven", "shadowfit", "orbits", "rk4",
-                             "offline"])
+                             "offline", "shader"])
     ap.add_argument("--orbits", type=int, default=65536, help="orbits: number of orbits")
     ap.add_argument("--proper-time", type=float, default=1.0e4, help="orbits: proper time")
     ap.add_argument("--fit-size", type=int, default=512, help="shadowfit: image side")
@@ -665,6 +714,11 @@ def main(argv=None) -> int:
         res = run_orbits(args)
         print(json.dumps({"metric": METRIC, "workload": "timelike orbits", "mode": args.mode,
                           **res}), flush=True)
+        return 0
+    if args.workload == "shader":
+        res = run_shader(args)
+        print(json.dumps({"metric": "fixed-step shader algorithm and image stage, 1920x1080",
+                          "workload": "shader", **res}), flush=True)
         return 0
     if args.workload == "offline":
         res = run_offline(args)
