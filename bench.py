@@ -368,6 +368,22 @@ def run_rk4(args) -> dict:
                                 ray_steps_per_s=ray_steps / (t * 1e-3),
                                 pct_fp64_vector_peak=100.0 * ray_steps * RK4_FLOP_PER_STEP
                                 / (t * 1e-3) / (FP64_VECTOR_PEAK_TFLOPS * 1e12))
+        # the implicit midpoint rule (integrators.py:177-207) on the same rays:
+        # 50 steps of 4 fixed-point iterations each, fused in one launch
+        mms = []
+        for k in range(args.warmup + args.steps):
+            work.copy_(soa)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(cur)
+            _lib.check(lib.bh_midpoint_f64(1.0, 0.9, _lib.MODE_FAST, _lib.ptr(work), n,
+                                           _lib.ptr(h), n, 50, 4, stream), "midpoint")
+            e1.record(cur)
+            e1.synchronize()
+            if k >= args.warmup:
+                mms.append(e0.elapsed_time(e1))
+        tm = float(np.median(mms))
+        out[f"midpoint_rays_{n}"] = dict(ms=tm, ns_per_ray_step=tm * 1e6 / ray_steps,
+                                         ray_steps_per_s=ray_steps / (tm * 1e-3))
         print(f"[bench] rk4 {n} rays done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         from oracle import bh_oracle as orc
