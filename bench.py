@@ -250,11 +250,21 @@ def run_uneven(args) -> dict:
     class PerJob:
         """One plain launch per job, in job order on one stream."""
 
+        tune = None  # the plain kernel's launch shape for every job
+
         def launch(self, tune=None):
             for _, p in prepared:
-                p.launch()
+                p.launch(tune or self.tune)
 
     per_job = PerJob()
+    per_job_tune_info = None
+    if args.tune == "auto":
+        # the baseline gets its own best launch shape too
+        table = autotune.time_candidates(per_job, autotune.plain_candidates(), reps=2)
+        best, best_ms = autotune.select(table)
+        per_job.tune = best
+        per_job_tune_info = dict(best=best.__dict__, best_ms=best_ms, candidates=len(table))
+        print(f"[bench] autotune per-job plain: {best}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         plan.launch()
         per_job.launch()
@@ -283,6 +293,7 @@ def run_uneven(args) -> dict:
         ms_batched_graph=ms_batched_graph, ms_per_job_graph=ms_per_job_graph,
         speedup_graph=ms_per_job_graph / ms_batched_graph,
         tune=tune_info or plan.tune.__dict__,
+        tune_per_job=per_job_tune_info or tracer.Tune().__dict__,
         ray_steps=steps_b, attempts=att_b, rays=plan.total_rays,
         pct_fp64_vector_peak_batched=100.0 * att_b * FLOP_PER_ATTEMPT / (ms_batched * 1e-3)
         / (FP64_VECTOR_PEAK_TFLOPS * 1e12),
