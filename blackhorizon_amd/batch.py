@@ -25,7 +25,7 @@ import torch
 from . import _dev, _lib
 from .backend import Array
 from .kerr import KerrSpacetime
-from .tracer import TraceResult, Tune, make_params, resolve_mode
+from .tracer import TraceResult, Tune, check_max_steps, make_params, resolve_mode
 
 MAX_JOBS_PER_LAUNCH = 256
 
@@ -69,6 +69,8 @@ class _Group:
     def __init__(self, jobs, idx, mode, dev, order_fn):
         self.idx = idx
         self.mode = mode
+        for i in idx:
+            check_max_steps(jobs[i].max_steps)
         states = [_dev.as_device_f64(jobs[i].state0, (8,), "state0") for i in idx]
         counts = [s.shape[0] for s in states]
         self.offsets_host = numpy.concatenate([[0], numpy.cumsum(counts)]).astype(numpy.int64)

@@ -87,13 +87,19 @@ static int32_t check_mode(int32_t mode) {
   return BH_OK;
 }
 
+constexpr int64_t MAX_STEPS_ABS = (int64_t)1 << 60;
+
 static int32_t check_params(const bh_trace_params *p) {
   if (!p) return fail(BH_EINVAL, "params is NULL");
   int32_t rc = check_spacetime(p->mass, p->spin);
   if (rc) return rc;
   if (p->mode != BH_MODE_FAST && p->mode != BH_MODE_PRECISE)
     return fail(BH_EINVAL, "mode must be BH_MODE_FAST or BH_MODE_PRECISE");
-  if (p->max_steps > (int64_t)1 << 60) return fail(BH_EINVAL, "max_steps too large");
+  // |max_steps| <= 2^60 keeps the attempt cap 4 * max_steps inside int64; a
+  // negative max_steps traces nothing (every ray MAX_STEPS), as the
+  // reference's loop bound 4 * max_steps <= 0 runs no iteration.
+  if (p->max_steps > MAX_STEPS_ABS || p->max_steps < -MAX_STEPS_ABS)
+    return fail(BH_EINVAL, "max_steps out of range [-2^60, 2^60]");
   return BH_OK;
 }
 

@@ -70,10 +70,25 @@ def resolve_mode(mode: str, rtol: float) -> str:
     return mode
 
 
+MAX_STEPS_ABS = 1 << 60
+
+
+def check_max_steps(max_steps) -> int:
+    """max_steps as an int, refused (ValueError) outside [-2^60, 2^60]: the
+    kernels cap attempts at 4 * max_steps in int64 (tracer.py:101).  The C
+    ABI refuses such a value for one job; a batched launch reads its job
+    table on the device, so the host checks every job."""
+    max_steps = int(max_steps)
+    if not -MAX_STEPS_ABS <= max_steps <= MAX_STEPS_ABS:
+        raise ValueError("max_steps must lie in [-2**60, 2**60]")
+    return max_steps
+
+
 def make_params(spacetime: KerrSpacetime, escape_radius: float, max_steps: int, rtol: float,
                 atol: float, initial_step: float, max_step: float, min_step: float,
                 capture_margin: float, mode: str = "auto",
                 lambda_end: float = math.inf) -> _lib.TraceParams:
+    max_steps = check_max_steps(max_steps)
     # tracer.py:84 computes the capture radius in host double precision
     capture_radius = spacetime.outer_horizon_radius * (1.0 + capture_margin)
     return _lib.TraceParams(spacetime.mass, spacetime.spin, float(escape_radius),

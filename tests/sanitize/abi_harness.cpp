@@ -124,7 +124,22 @@ std::vector<Case> build_cases() {
     bh_trace_params p = good_params();
     p.max_steps = (int64_t)1 << 62;
     return bh_count_outside_f64(&p, d, 4, 4, i32, nullptr);
-  }, "max_steps too large");
+  }, "max_steps out of range");
+  add("count: max_steps -2^62", [=] {
+    bh_trace_params p = good_params();
+    p.max_steps = -((int64_t)1 << 62);
+    return bh_count_outside_f64(&p, d, 4, 4, i32, nullptr);
+  }, "max_steps out of range");
+  add("count: max_steps INT64_MIN", [=] {
+    bh_trace_params p = good_params();
+    p.max_steps = INT64_MIN;
+    return bh_count_outside_f64(&p, d, 4, 4, i32, nullptr);
+  }, "max_steps out of range");
+  add("plain: max_steps -2^61", [=] {
+    bh_trace_params p = good_params();
+    p.max_steps = -((int64_t)1 << 61);
+    return bh_trace_plain_f64(&p, nullptr, d, 4, d, 4, i32, i64, i64, u64, nullptr, 4, nullptr);
+  }, "max_steps out of range");
   add("count: n -1", [=] {
     bh_trace_params p = good_params();
     return bh_count_outside_f64(&p, d, 4, -1, i32, nullptr);

@@ -41,6 +41,36 @@ def test_invalid_params_rejected(over, msg):
         _lib.check(rc, "trace")
 
 
+@pytest.mark.parametrize("max_steps", [(1 << 60) + 1, -(1 << 60) - 1, 1 << 62, -(1 << 63)])
+def test_max_steps_range_rejected(max_steps):
+    """|max_steps| beyond 2^60 would overflow the attempt cap 4 * max_steps:
+    the ABI refuses it for one job, and the host refuses it for every job
+    (the batched launch reads its job table on the device)."""
+    from blackhorizon_amd import batch, tracer
+    from blackhorizon_amd.kerr import KerrSpacetime
+
+    lib = _lib.load()
+    rc = lib.bh_trace_plain_f64(_params(max_steps=max_steps), None, None, 0, None, 0, None, None,
+                                None, None, None, 0, None)
+    assert rc == _lib.BH_EINVAL
+    assert "max_steps out of range" in lib.bh_last_error().decode()
+    st = KerrSpacetime(spin=0.5)
+    with pytest.raises(ValueError, match="max_steps"):
+        tracer.make_params(st, 45.0, max_steps, 1e-8, 1e-10, 0.1, 2.0, 1e-10, 1e-3)
+    with pytest.raises(ValueError, match="max_steps"):
+        batch._Group([batch.TraceJob(st, None, 45.0, max_steps=max_steps)], [0], "fast", None,
+                     None)
+
+
+def test_negative_max_steps_accepted():
+    """A negative max_steps is valid (the reference's loop then runs no
+    iteration): only the range is refused."""
+    lib = _lib.load()
+    rc = lib.bh_trace_plain_f64(_params(max_steps=-5), None, None, 0, None, 0, None, None, None,
+                                None, None, 0, None)
+    assert rc == _lib.BH_OK
+
+
 def test_invalid_tune_rejected():
     lib = _lib.load()
     for tune in (_lib.TuneParams(96, 1, 1, 0), _lib.TuneParams(512, 1, 1, 0),
