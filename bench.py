@@ -496,6 +496,9 @@ def run_shadowfit(args) -> dict:
     """512 x 512 differentiable shadow fit at a = 0.7, inclination 45 deg,
     r_obs = 40 M, FOV 30 deg, escape sphere 45 M, rtol 1e-8, checkpoints
     every 32 accepted steps; target traced at a = 0.75, inclination 50 deg.
+    Loss: squared angle of the escape directions on the escape sphere for
+    rays escaped in both, of the plunge directions on the scoring sphere
+    for rays captured in both, the squared capture margin for the others.
     Times the plain forward trace, the recording forward, the adjoint op
     (Recorded.backward for the loss's terminal adjoint) and the whole fit
     step (camera rays, recording trace, loss and autograd gradients)."""
@@ -515,7 +518,7 @@ def run_shadowfit(args) -> dict:
     with torch.no_grad():
         tgt = adjoint.camera_state(0.75, 50.0, 40.0, 30.0, size, size)
         Yt, st_t = adjoint.trace_terminal(tgt, 0.75, cfg_t)
-        dirs_t = adjoint.terminal_directions(Yt)
+        dirs_t = adjoint.terminal_directions(Yt, 0.75)
     p = [torch.tensor(v, dtype=torch.float64, device="cuda", requires_grad=True)
          for v in (0.7, 45.0, 40.0)]
     y0 = adjoint.camera_state(p[0], p[1], p[2], 30.0, size, size)
@@ -528,7 +531,7 @@ def run_shadowfit(args) -> dict:
 
     # the terminal adjoint of the fit loss, for timing the adjoint op alone
     Y, status = adjoint.trace_terminal(y0, p[0], cfg)
-    loss = adjoint.shadow_loss(Y, status, dirs_t, st_t)
+    loss = adjoint.shadow_loss(Y, status, p[0], dirs_t, st_t, state0=y0)
     gY = torch.autograd.grad(loss, Y, retain_graph=True)[0].detach()
     y0d = y0.detach().contiguous()
     times = {"plain": [], "record": [], "adjoint": [], "fit_step": []}
@@ -546,7 +549,7 @@ def run_shadowfit(args) -> dict:
         # the whole differentiable fit step: camera rays, trace, loss, gradients
         y0 = adjoint.camera_state(p[0], p[1], p[2], 30.0, size, size)
         Y, status = adjoint.trace_terminal(y0, p[0], cfg)
-        loss = adjoint.shadow_loss(Y, status, dirs_t, st_t)
+        loss = adjoint.shadow_loss(Y, status, p[0], dirs_t, st_t, state0=y0)
         g = torch.autograd.grad(loss, p)
         e[4].record(cur)
         torch.cuda.synchronize()

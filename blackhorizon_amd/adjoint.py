@@ -253,26 +253,133 @@ def camera_state(spin, inclination_degrees, distance, fov_degrees: float, width:
 # shadow-fit loss
 # ---------------------------------------------------------------------------
 
-def shadow_loss(terminal: torch.Tensor, status: torch.Tensor, target_dirs: torch.Tensor,
-                target_status: torch.Tensor, mismatch_cost: float = 1.0,
-                classes: tuple = (1, 2)) -> torch.Tensor:
-    """Sum over rays of |n - n_target|^2 with n the unit position of the
-    terminal point (the squared chord between the two directions, the smooth
-    form of the squared angular distance), for rays whose class matches the
-    target's: escaped rays by where they leave through the escape sphere,
-    captured rays by where they plunge through the scoring sphere near the
-    horizon; rays whose class differs cost ``mismatch_cost`` each.
-    ``classes`` limits the scored classes (1 captured, 2 escaped)."""
+def ks_radius_torch(spin, pos: torch.Tensor) -> torch.Tensor:
+    """Kerr-Schild radius of positions (..., 3) (kerr.py:136-142), differentiable."""
+    x, y, z = pos[..., 0], pos[..., 1], pos[..., 2]
+    a2 = spin * spin
+    q = x * x + y * y + z * z - a2
+    s = torch.clamp(torch.sqrt(q * q + 4.0 * a2 * z * z), min=TINY)
+    return torch.sqrt(torch.clamp(0.5 * (q + s), min=TINY))
+
+
+def terminal_directions(terminal: torch.Tensor, spin) -> torch.Tensor:
+    """Unit vectors (sin th cos ph, sin th sin ph, cos th) of the spheroidal
+    angles (th, ph) of each terminal point on its sphere r = const, from
+    x + i y = (r + i a) sin th e^{i ph}, z = r cos th: the spatial part of
+    the Kerr-Schild null vector l at that point (kerr.py:147-152),
+    differentiable in the point and the spin."""
     pos = terminal[:, 1:4]
-    n = pos / torch.linalg.norm(pos, dim=-1, keepdim=True)
+    r = ks_radius_torch(spin, pos)
+    w = r * r + spin * spin
+    x, y, z = pos[:, 0], pos[:, 1], pos[:, 2]
+    return torch.stack([(r * x + spin * y) / w, (r * y - spin * x) / w, z / r], dim=-1)
+
+
+def angle_sq(n: torch.Tensor, m: torch.Tensor) -> torch.Tensor:
+    """Squared great-circle angle between unit vectors, psi^2 with
+    psi = 2 asin(|n - m| / 2), smooth through psi = 0 (series below
+    |n - m|^2 = 1e-12)."""
+    c2 = ((n - m) ** 2).sum(-1)
+    big = 4.0 * torch.asin(0.5 * torch.sqrt(torch.clamp(c2, min=1e-24))) ** 2
+    return torch.where(c2 < 1e-12, c2 * (1.0 + c2 / 12.0), big)
+
+
+def _radial_potential(r, a, mass, lam, eta):
+    """Kerr radial potential of a photon per E^4, normalised by (r^2 + a^2)^2:
+    R(r) = (r^2 + a^2 - a lam)^2 - Delta (eta + (lam - a)^2),
+    Delta = r^2 - 2 M r + a^2 (Carter 1968)."""
+    w = r * r + a * a
+    delta = r * r - 2.0 * mass * r + a * a
+    return ((w - a * lam) ** 2 - delta * (eta + (lam - a) ** 2)) / (w * w)
+
+
+def capture_margin(state0: torch.Tensor, spin, mass: float = 1.0, grid: int = 256,
+                   refine: int = 60) -> torch.Tensor:
+    """Smooth horizon-proximity score of photon states (n, 8): the minimum of
+    the normalised radial potential R(r) / (r^2 + a^2)^2 over
+    r_+ <= r <= r_0, from the ray's constants of motion E = -p_t,
+    L_z = x p_y - y p_x and Carter's Q = p_th^2 + cos^2 th (L_z^2 / sin^2 th
+    - a^2 E^2), with lam = L_z / E, eta = Q / E^2 and, in Kerr-Schild
+    coordinates, p_th = cot th (x p_x + y p_y) - r sin th p_z.  An ingoing
+    photon reaches the horizon iff R stays positive (score > 0): the score
+    is positive inside the shadow, negative outside and zero on its edge
+    (a = 0: 1 - b^2 / 27).  The minimiser r* is found on detached values
+    (grid, then golden-section search); the score is R(r*) with the graph,
+    whose derivative is the true one at an interior minimum (dR/dr = 0
+    there), and r* = r_+(a) is kept differentiable at the horizon end."""
+    a = spin if isinstance(spin, torch.Tensor) else torch.tensor(float(spin), dtype=torch.float64,
+                                                                 device=state0.device)
+    pos = state0[:, 1:4]
+    x, y, z = pos[:, 0], pos[:, 1], pos[:, 2]
+    p_t, px, py, pz = state0[:, 4], state0[:, 5], state0[:, 6], state0[:, 7]
+    r = ks_radius_torch(a, pos)
+    cos_t = z / r
+    sin2 = torch.clamp(1.0 - cos_t * cos_t, min=TINY)
+    sin_t = torch.sqrt(sin2)
+    p_th = cos_t / sin_t * (x * px + y * py) - r * sin_t * pz
+    energy = -p_t
+    l_z = x * py - y * px
+    q = p_th * p_th + cos_t * cos_t * (l_z * l_z / sin2 - a * a * energy * energy)
+    lam = l_z / energy
+    eta = q / (energy * energy)
+    r_plus = mass + torch.sqrt(mass * mass - a * a)
+    with torch.no_grad():
+        ad, lamd, etad = a.detach(), lam.detach(), eta.detach()
+        lo_r = r_plus.detach().expand_as(r)
+        hi_r = r.detach()
+        u = torch.linspace(0.0, 1.0, grid, dtype=torch.float64, device=state0.device)
+        rr = lo_r[:, None] + (hi_r - lo_r)[:, None] * u[None, :]
+        vals = _radial_potential(rr, ad, mass, lamd[:, None], etad[:, None])
+        k = torch.argmin(vals, dim=1)
+        lo = rr.gather(1, torch.clamp(k - 1, min=0)[:, None])[:, 0]
+        hi = rr.gather(1, torch.clamp(k + 1, max=grid - 1)[:, None])[:, 0]
+        g = (5.0 ** 0.5 - 1.0) / 2.0
+        for _ in range(refine):
+            c = hi - g * (hi - lo)
+            d = lo + g * (hi - lo)
+            left = (_radial_potential(c, ad, mass, lamd, etad)
+                    < _radial_potential(d, ad, mass, lamd, etad))
+            hi = torch.where(left, d, hi)
+            lo = torch.where(left, lo, c)
+        r_star = 0.5 * (lo + hi)
+        at_horizon = k == 0
+    r_eval = torch.where(at_horizon, r_plus.expand_as(r_star), r_star)
+    return _radial_potential(r_eval, a, mass, lam, eta)
+
+
+def shadow_loss(terminal: torch.Tensor, status: torch.Tensor, spin, target_dirs: torch.Tensor,
+                target_status: torch.Tensor, state0: torch.Tensor | None = None,
+                classes: tuple = (1, 2), margin_weight: float = 1.0,
+                mass: float = 1.0) -> torch.Tensor:
+    """Shadow-fit loss, summed over rays:
+
+    * escaped in both the trace and the target: psi^2, the squared angle
+      between the (th, ph) where the ray leaves through the escape sphere
+      and the target's (``terminal_directions``);
+    * captured in both: psi^2 between the (th, ph) where the ray plunges
+      through the scoring sphere r = score_factor r_+ near the horizon and
+      the target's;
+    * captured in one and escaped in the other (needs ``state0``):
+      ``margin_weight`` * m^2 with m the ray's ``capture_margin``, a smooth
+      horizon-proximity term that vanishes on the shadow's edge and pulls
+      the ray towards the target's class;
+    * any other status: 0.
+
+    ``classes`` limits the matched classes scored (1 captured, 2 escaped);
+    the mismatch term is scored when both are listed and ``state0`` is
+    given.  ``target_dirs`` are ``terminal_directions`` of the target's
+    terminal points."""
+    n = terminal_directions(terminal, spin)
+    zero = torch.zeros(status.shape[0], dtype=torch.float64, device=terminal.device)
     scored = torch.zeros_like(status, dtype=torch.bool)
     for c in classes:
         scored |= status == c
     same = (status == target_status) & scored
-    per_ray = ((n - target_dirs) ** 2).sum(-1)
-    return torch.where(same, per_ray, torch.full_like(per_ray, mismatch_cost)).sum()
-
-
-def terminal_directions(terminal: torch.Tensor) -> torch.Tensor:
-    pos = terminal[:, 1:4]
-    return pos / torch.linalg.norm(pos, dim=-1, keepdim=True)
+    per_ray = torch.where(same, angle_sq(n, target_dirs), zero)
+    if state0 is not None and 1 in classes and 2 in classes:
+        mism = (((status == 1) & (target_status == 2)) | ((status == 2) & (target_status == 1)))
+        if bool(mism.any()):
+            idx = torch.nonzero(mism)[:, 0]
+            m = capture_margin(state0[idx], spin, mass)
+            per_ray = per_ray.index_add(0, idx, margin_weight * m * m)
+    return per_ray.sum()
