@@ -350,7 +350,7 @@ def capture_margin(state0: torch.Tensor, spin, mass: float = 1.0, grid: int = 25
 def shadow_loss(terminal: torch.Tensor, status: torch.Tensor, spin, target_dirs: torch.Tensor,
                 target_status: torch.Tensor, state0: torch.Tensor | None = None,
                 classes: tuple = (1, 2), margin_weight: float = 1.0,
-                mass: float = 1.0) -> torch.Tensor:
+                mass: float = 1.0, ray_weights: torch.Tensor | None = None) -> torch.Tensor:
     """Shadow-fit loss, summed over rays:
 
     * escaped in both the trace and the target: psi^2, the squared angle
@@ -368,7 +368,7 @@ def shadow_loss(terminal: torch.Tensor, status: torch.Tensor, spin, target_dirs:
     ``classes`` limits the matched classes scored (1 captured, 2 escaped);
     the mismatch term is scored when both are listed and ``state0`` is
     given.  ``target_dirs`` are ``terminal_directions`` of the target's
-    terminal points."""
+    terminal points; optional ``ray_weights`` (n,) scale each ray's term."""
     n = terminal_directions(terminal, spin)
     zero = torch.zeros(status.shape[0], dtype=torch.float64, device=terminal.device)
     scored = torch.zeros_like(status, dtype=torch.bool)
@@ -382,4 +382,6 @@ def shadow_loss(terminal: torch.Tensor, status: torch.Tensor, spin, target_dirs:
             idx = torch.nonzero(mism)[:, 0]
             m = capture_margin(state0[idx], spin, mass)
             per_ray = per_ray.index_add(0, idx, margin_weight * m * m)
+    if ray_weights is not None:
+        per_ray = per_ray * ray_weights
     return per_ray.sum()

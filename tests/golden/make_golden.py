@@ -130,6 +130,60 @@ def edge_cases(ref_trace, ref_kerr):
     np.savez_compressed(os.path.join(HERE, "edge_cases.npz"), **out)
 
 
+def edge_cases_r8(ref_trace, ref_kerr):
+    """The controller's failure and cap outcomes (tracer.py:101-154) at
+    rtol 1e-8, atol 1e-10, where the fast build runs: non-finite momenta,
+    time and positions (FAILED), max_steps 0, negative, 1 and 7 (MAX_STEPS
+    by accepted steps), an initial step so large that some rays hit the
+    4 max_steps attempt cap first, h below min_step (FAILED), a small
+    initial and maximum step, and a huge one."""
+    st = orc.Kerr(spin=0.9)
+    pos = np.array([[20.0, 1.0, 2.0], [20.0, 0.0, 0.0], [10.0, 3.0, -4.0],
+                    [1.2, 0.1, 0.3], [20.0, 0.5, 0.5], [8.0, -2.0, 1.0]])
+    d = np.array([[-1.0, 0.1, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.2],
+                  [-1.0, 0.0, 0.0], [-1.0, 0.0, 0.05], [0.3, 1.0, -0.1]])
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    mom = orc.null_momentum_from_velocity(st, np.where(np.arange(6)[:, None] == 3,
+                                                       np.array([[5.0, 0.1, 0.3]]), pos), d, "past")
+    base = orc.build_state(pos, mom)
+    bad = base.copy()
+    bad[0, 5] = np.nan   # momentum NaN
+    bad[2, 6] = np.inf   # momentum inf
+    bad[4, 0] = np.nan   # coordinate time NaN: RHS finite, state not
+    bad_pos = base.copy()
+    bad_pos[1, 1] = np.nan    # position NaN: r0 NaN passes the r0 check
+    bad_pos[3, 7] = -np.inf   # momentum -inf on the ray inside the horizon
+    bad_pos[5, 4] = np.nan    # p_t NaN
+    r8 = dict(escape_radius=40.0, rtol=1e-8, atol=1e-10)
+    cases = {
+        "default": (base, {}),
+        "nonfinite": (bad, {}),
+        "nonfinite_pos": (bad_pos, {}),
+        "max_steps_0": (base, dict(max_steps=0)),
+        "max_steps_neg": (base, dict(max_steps=-3)),
+        "max_steps_1": (base, dict(max_steps=1)),
+        "max_steps_7": (base, dict(max_steps=7)),
+        "attempt_cap": (base, dict(max_steps=2, initial_step=1e4, max_step=1e4)),
+        "min_step_high": (base, dict(min_step=0.5)),
+        "small_h": (base, dict(initial_step=1e-3, max_step=0.5)),
+        "huge_h": (base, dict(initial_step=1e3, max_step=1e3, max_steps=50)),
+    }
+    out = {}
+    for name, (s0, over) in cases.items():
+        kw = dict(r8)
+        kw.update(over)
+        res = ref_trace(ref_kerr(spin=0.9), s0, **kw)
+        out[name + "_state0"] = s0
+        out[name + "_states"] = np.asarray(res.states)
+        out[name + "_status"] = np.asarray(res.status)
+        out[name + "_steps"] = np.asarray(res.steps)
+        out[name + "_iterations"] = np.int64(res.iterations)
+        for k, v in kw.items():
+            out[name + "_kw_" + k] = np.float64(v)
+        print(name, res.status, res.steps, res.iterations)
+    np.savez_compressed(os.path.join(HERE, "edge_cases_r8.npz"), **out)
+
+
 def uneven_subsample(ref_trace, ref_kerr, n_jobs=8, per_job=48):
     """A few rays from each of the first jobs of the uneven mix (mixed
     spins of both signs, distances and tolerances)."""
@@ -228,9 +282,65 @@ def dense_orbits(integrate_dense, geodesic_rhs, ref_kerr):
     np.savez_compressed(os.path.join(HERE, "dense_orbits.npz"), **out)
 
 
+def c5_orbits(integrate_dense, geodesic_rhs, ref_kerr, n_pick=8):
+    """Seeded orbits from inside the timelike-orbit workload (65,536
+    near-circular orbits around a = 0.9 in 16 jobs, proper time 1e4 M,
+    rtol 1e-10), each integrated alone by the reference's integrate_dense
+    (integrators.py:127-174) with that workload's settings (its defaults:
+    atol 1e-12, initial step 0.05, max step 2).  ``spread`` is how far the
+    final state moves under ulp-level step-factor changes (the oracle with
+    libm pow and with random-ulp pow), the conditioning of each orbit's
+    result."""
+    import warnings
+
+    jobs = workloads.orbit_batches()
+    pos = np.concatenate([j.positions for j in jobs])
+    vel = np.concatenate([j.velocities for j in jobs])
+    st = orc.Kerr(spin=0.9)
+    kst = ref_kerr(spin=0.9)
+    rng = np.random.default_rng(2024)
+    out = {}
+    k = 0
+    for i in rng.permutation(len(pos)):
+        if k == n_pick:
+            break
+        s0 = orc.build_state(pos[i:i + 1], orc.timelike_momentum(st, pos[i:i + 1],
+                                                                 vel[i:i + 1]))
+        try:
+            states = integrate_dense(lambda y: geodesic_rhs(kst, y), s0, lambda_total=1.0e4,
+                                     rtol=1e-10)
+        except RuntimeError as e:
+            print("skipped", i, e)
+            continue
+        final = np.asarray(states[-1])
+        if np.linalg.norm(final[0, 1:4]) > 900.0 or np.linalg.norm(final[0, 1:4]) < 2.0:
+            print("skipped", i, "(escapes or plunges)")
+            continue
+        spread = 0.0
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore", RuntimeWarning)
+            for fn in (orc.step_factor_libm, orc.step_factor_ulp_noise(1),
+                       orc.step_factor_ulp_noise(2)):
+                alt = orc.integrate_dense(lambda y: orc.geodesic_rhs(st, y), s0, 1.0e4,
+                                          rtol=1e-10, factor_fn=fn)[-1]
+                d = np.max([np.linalg.norm(alt[0, b] - final[0, b]) /
+                            max(np.linalg.norm(final[0, b]), 1.0)
+                            for b in (slice(0, 1), slice(1, 4), slice(4, 5), slice(5, 8))])
+                spread = max(spread, float(d))
+        out[f"o{k}_index"] = np.int64(i)
+        out[f"o{k}_state0"] = s0
+        out[f"o{k}_final"] = final
+        out[f"o{k}_steps"] = np.int64(len(states) - 1)
+        out[f"o{k}_spread"] = np.float64(spread)
+        print("c5 orbit", k, "index", i, "steps", len(states) - 1, "spread", spread, flush=True)
+        k += 1
+    np.savez_compressed(os.path.join(HERE, "c5_orbits.npz"), **out)
+
+
 def main():
     ref_root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
-    parts = sys.argv[2:] or ["png", "edge", "fans", "small", "uneven", "hd", "dense"]
+    parts = sys.argv[2:] or ["png", "edge", "edge_r8", "fans", "small", "uneven", "hd", "dense",
+                             "c5"]
     sys.path.insert(0, ref_root)
     from blackhorizon.geodesics import geodesic_rhs
     from blackhorizon.integrators import integrate_dense
@@ -241,6 +351,8 @@ def main():
         shadow_png_classes(ref_root)
     if "edge" in parts:
         edge_cases(trace_rays, KerrSpacetime)
+    if "edge_r8" in parts:
+        edge_cases_r8(trace_rays, KerrSpacetime)
     if "fans" in parts:
         fans(trace_rays, KerrSpacetime)
     if "small" in parts:
@@ -252,6 +364,8 @@ def main():
         hd_subsample(trace_rays, KerrSpacetime)
     if "dense" in parts:
         dense_orbits(integrate_dense, geodesic_rhs, KerrSpacetime)
+    if "c5" in parts:
+        c5_orbits(integrate_dense, geodesic_rhs, KerrSpacetime)
 
 
 if __name__ == "__main__":

@@ -115,34 +115,69 @@ def _fit_target(size=24, mode="auto", K=8):
     return adjoint.terminal_directions(Yt, 0.75), st_t, cfg
 
 
-def _loss(params, target, cfg, classes=(1, 2), size=24):
+def _loss(params, target, cfg, classes=(1, 2), size=24, weights=None):
     from blackhorizon_amd import adjoint
 
     a, inc, dist = params
     y0 = adjoint.camera_state(a, inc, dist, FOV, size, size)
     Y, st = adjoint.trace_terminal(y0, a, cfg)
-    return adjoint.shadow_loss(Y, st, a, target[0], target[1], state0=y0, classes=classes)
+    return adjoint.shadow_loss(Y, st, a, target[0], target[1], state0=y0, classes=classes,
+                               ray_weights=weights)
 
 
-def _fd_check(target, cfg, classes, params_to_check, tol, size=24, h=1e-6):
+def _smooth_rays(values, k, h, cfg, size):
+    """Rays whose status and accepted-step count are the same at the
+    parameters and at +-h in parameter k: there every ray's trace is one
+    smooth piece of the adaptive map.  Elsewhere a step is accepted or
+    rejected differently inside the interval (the escape step lands on the
+    other side of the sphere, say) and the map jumps, so central
+    differences across it are no derivative."""
+    from blackhorizon_amd import adjoint
+
+    outs = []
+    for sgn in (0, 1, -1):
+        v = list(values)
+        v[k] += sgn * h
+        with torch.no_grad():
+            rec = adjoint.Recorded(adjoint.camera_state(v[0], v[1], v[2], FOV, size, size), v[0],
+                                   cfg)
+            outs.append((rec.status.clone(), rec.steps.clone()))
+    keep = torch.ones_like(outs[0][0], dtype=torch.bool)
+    for st, steps in outs[1:]:
+        keep &= (st == outs[0][0]) & (steps == outs[0][1])
+    return keep
+
+
+def _fd_check(target, cfg, classes, params_to_check, tol, size=24, h=1e-6,
+              max_excluded=0.001):
+    """Adjoint vs central differences of the loss over the rays that stay
+    on one smooth piece of the adaptive map (counted, printed, and at most
+    ``max_excluded`` of the rays left out)."""
     base = [SPIN, INC, DIST]
-    p = [torch.tensor(v, dtype=torch.float64, device="cuda", requires_grad=True) for v in base]
-    L = _loss(p, target, cfg, classes, size)
-    L.backward()
-    grads = [float(t.grad) for t in p]
+    grads = []
     for k in params_to_check:
+        keep = _smooth_rays(base, k, h, cfg, size)
+        n_out = int((~keep).sum())
+        w = keep.to(torch.float64)
+        p = [torch.tensor(v, dtype=torch.float64, device="cuda", requires_grad=True)
+             for v in base]
+        _loss(p, target, cfg, classes, size, w).backward()
+        g = float(p[k].grad)
         with torch.no_grad():
             up, dn = list(base), list(base)
             up[k] += h
             dn[k] -= h
             lp = float(_loss([torch.tensor(v, dtype=torch.float64, device="cuda") for v in up],
-                             target, cfg, classes, size))
+                             target, cfg, classes, size, w))
             lm = float(_loss([torch.tensor(v, dtype=torch.float64, device="cuda") for v in dn],
-                             target, cfg, classes, size))
+                             target, cfg, classes, size, w))
         fd = (lp - lm) / (2 * h)
-        rel = abs(fd - grads[k]) / abs(fd)
-        print(f"param {k}: fd {fd:.12g} adjoint {grads[k]:.12g} rel {rel:.2e}")
-        assert rel <= tol, (k, fd, grads[k])
+        rel = abs(fd - g) / abs(fd)
+        print(f"param {k}: fd {fd:.12g} adjoint {g:.12g} rel {rel:.2e}; {n_out} of "
+              f"{keep.numel()} rays change step count or status within +-h")
+        assert n_out <= max_excluded * keep.numel()
+        assert rel <= tol, (k, fd, g)
+        grads.append(g)
     return grads
 
 
@@ -220,7 +255,10 @@ def test_adjoint_equals_frozen_step_differences(hip):
 def test_full_size_fit_gradients(hip):
     """The 512 x 512 fit with checkpoints every 32 accepted steps: gradients
     bitwise equal to K = 8, and the spin derivative of the full loss equal to
-    central differences (h = 1e-6) to 1e-5."""
+    central differences (h = 1e-6) to 1e-5 over the rays whose step count
+    stays put within +-h (all but ~55 of 262,144; the few near-critical
+    escaped rays that take one more or one fewer step have gradients up to
+    1e5, and one of them alone moves the sum by 2e-4)."""
     from blackhorizon_amd import adjoint
 
     size = 512
@@ -234,8 +272,7 @@ def test_full_size_fit_gradients(hip):
         grads[K] = [float(t.grad) for t in p]
         print(f"K = {K}: loss {float(L):.12g} grads {grads[K]}")
     assert grads[32] == grads[8]
-    g = _fd_check((dirs, st_t), cfg32, (1, 2), (0,), 1e-5, size=size)
-    assert g == grads[32]
+    _fd_check((dirs, st_t), cfg32, (1, 2), (0,), 1e-5, size=size)
 
 
 def test_per_ray_momentum_gradients_match_central_differences(hip):

@@ -3,6 +3,8 @@ time) against the reference's integrate_dense run per ray (golden vectors),
 batched jobs against plain launches, and the adjoint with an affine end
 against the autograd oracle."""
 
+import math
+
 import numpy as np
 import pytest
 import torch
@@ -361,3 +363,207 @@ def test_debris_evolves_on_valid_geodesics(hip):
                                                   device="cuda"), 400)
     np.testing.assert_allclose(orc.hamiltonian(orc.Kerr(spin=0.9), out.cpu().numpy()), -0.5,
                                atol=1e-5)
+
+
+# ---------------------------------------------------------------------------
+# the reference's orbit checks (test_orbits.py) on device trajectories
+# ---------------------------------------------------------------------------
+
+def _trajectory(hip, spin, s0, lam, rtol):
+    from blackhorizon_amd import integrators
+
+    states = integrators.integrate_dense(hip.KerrSpacetime(spin=spin), s0, lambda_total=lam,
+                                         rtol=rtol)
+    return np.concatenate(states, axis=0)
+
+
+def test_photon_sphere_orbit_on_device(hip):
+    """test_orbits.py:68-91: a tangential photon at r = 3 M (a = 0) rides
+    the unstable circular orbit for 1.5 turns with r within 1e-6 of 3 M at
+    every accepted step of integrate_dense at rtol 1e-12, and winds by more
+    than 2.5 pi; its impact parameter is 3 sqrt 3 (test_orbits.py:93-104)."""
+    st = orc.Kerr(spin=0.0)
+    pos = np.array([[3.0, 0.0, 0.0]])
+    mom = orc.null_momentum_from_velocity(st, pos, np.array([[0.0, 1.0, 0.0]]), "future")
+    s0 = orc.build_state(pos, mom)
+    e, lz = orc.conserved_quantities(s0)
+    np.testing.assert_allclose(lz[0] / e[0], 3.0 * math.sqrt(3.0), rtol=1e-12)
+    traj = _trajectory(hip, 0.0, s0, 16.0, 1e-12)
+    r = st.kerr_schild_radius(traj[:, 1], traj[:, 2], traj[:, 3])
+    print(f"photon sphere: {traj.shape[0] - 1} steps, max |r - 3| {np.max(np.abs(r - 3.0)):.2e}")
+    np.testing.assert_allclose(r, 3.0, atol=1e-6)
+    phi = np.unwrap(np.arctan2(traj[:, 2], traj[:, 1]))
+    assert abs(phi[-1] - phi[0]) > 2.5 * np.pi
+
+
+def test_reference_orbit_checks_on_device_trajectories(hip):
+    """test_orbits.py:22-64 and 108-136 on every accepted state of the
+    device's integrate_dense: the a = 0.9 strong-field photon flyby (E to
+    1e-12, L_z to 1e-8, H to 1e-9, closest approach below 6 M), the
+    timelike normalisation H = -1/2 to 1e-9, and the r = 6 M prograde
+    circular orbit (r to 1e-6, z to 1e-8, E to 1e-12, L_z to 1e-10)."""
+    st = orc.Kerr(spin=0.9)
+    pos = np.array([[30.0, 4.0, 1.5]])
+    mom = orc.null_momentum_from_velocity(st, pos, np.array([[-1.0, 0.0, 0.0]]), "future")
+    traj = _trajectory(hip, 0.9, orc.build_state(pos, mom), 70.0, 1e-11)
+    e, lz = orc.conserved_quantities(traj)
+    np.testing.assert_allclose(e, e[0], rtol=1e-12)
+    np.testing.assert_allclose(lz, lz[0], rtol=1e-8)
+    np.testing.assert_allclose(orc.hamiltonian(st, traj), 0.0, atol=1e-9)
+    assert float(st.kerr_schild_radius(traj[:, 1], traj[:, 2], traj[:, 3]).min()) < 6.0
+    om = 1.0 / (6.0 ** 1.5 + 0.9)
+    pos = np.array([[6.0, 0.9, 0.0]])
+    s0 = orc.build_state(pos, orc.timelike_momentum(st, pos, np.array([[-om * 0.9, om * 6.0,
+                                                                          0.0]])))
+    traj = _trajectory(hip, 0.9, s0, 120.0, 1e-11)
+    np.testing.assert_allclose(orc.hamiltonian(st, traj), -0.5, rtol=1e-9)
+    traj = _trajectory(hip, 0.9, s0, 170.0, 1e-11)
+    np.testing.assert_allclose(st.kerr_schild_radius(traj[:, 1], traj[:, 2], traj[:, 3]), 6.0,
+                               rtol=1e-6)
+    assert np.allclose(traj[:, 3], 0.0, atol=1e-8)
+    e, lz = orc.conserved_quantities(traj)
+    np.testing.assert_allclose(e, e[0], rtol=1e-12)
+    np.testing.assert_allclose(lz, lz[0], rtol=1e-10)
+
+
+# ---------------------------------------------------------------------------
+# the timelike-orbit workload at its configured size
+# ---------------------------------------------------------------------------
+
+def test_timelike_orbit_workload_full_size(hip):
+    """65,536 near-circular orbits around a = 0.9 in 16 uneven jobs,
+    integrated to proper time 1e4 M at rtol 1e-10 in one persistent batched
+    plan (workloads.orbit_batches() defaults):
+
+    * every job's results equal one plain launch of that job, bit for bit;
+    * p_t is carried exactly (E drift 0);
+    * every orbit ends TERMINATED at tau = 1e4, CAPTURED, or ESCAPED, and
+      only orbits with E > 0.999 (unbound or apoapsis beyond ~1000 M)
+      escape; at least 97% reach tau;
+    * on the orbits that reach tau: relative L_z drift median below 1e-10
+      and max below 1e-5, |H + 1/2| drift median below 1e-11 and max below
+      1e-6 (measured: 2.9e-12 / 7.8e-7 and 1.7e-13 / 2.6e-8);
+    * seeded orbits inside the launch meet the parity bar against the
+      reference's integrate_dense run on each alone (c5_orbits.npz), within
+      10x the reference's own host-to-host spread where that exceeds 1e-7.
+    """
+    import time
+
+    import torch
+
+    from blackhorizon_amd import batch, geodesics, kerr
+    from blackhorizon_amd.tracer import integrate_rays
+
+    jobs = workloads.orbit_batches()
+    assert len(jobs) == 16 and sum(j.n_rays for j in jobs) == 65536
+    st = kerr.KerrSpacetime(1.0, 0.9)
+    tjobs, states = [], []
+    for j in jobs:
+        pos = torch.from_numpy(j.positions).cuda()
+        mom = geodesics.timelike_momentum(st, pos, torch.from_numpy(j.velocities).cuda())
+        s0 = geodesics.build_state(pos, mom)
+        states.append(s0)
+        tjobs.append(batch.TraceJob(st, s0, j.escape_radius, j.max_steps, j.rtol, j.atol,
+                                    j.initial_step, j.max_step, j.min_step, j.capture_margin,
+                                    j.lambda_end))
+    plan = batch.BatchPlan(tjobs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan.launch()
+    torch.cuda.synchronize()
+    t_batched = time.perf_counter() - t0
+    res = plan.results(as_numpy=False)
+    s0 = torch.cat(states)
+    s1 = torch.cat([r.states for r in res])
+    status = torch.cat([r.status for r in res]).cpu().numpy()
+    t0 = time.perf_counter()
+    for j, t, r in zip(jobs, tjobs, res):
+        alone = integrate_rays(st, t.state0, j.lambda_end, rtol=j.rtol, atol=j.atol,
+                               initial_step=j.initial_step, max_step=j.max_step,
+                               min_step=j.min_step, max_steps=j.max_steps,
+                               escape_radius=j.escape_radius, capture_margin=j.capture_margin)
+        assert torch.equal(r.status, alone.status)
+        assert torch.equal(r.steps, alone.steps)
+        assert torch.equal(r.states, alone.states)
+    t_plain = time.perf_counter() - t0
+    assert torch.equal(s1[:, 4], s0[:, 4])
+    counts = np.bincount(status, minlength=7)
+    assert counts[1] + counts[2] + counts[6] == 65536, counts
+    assert counts[6] >= 0.97 * 65536
+    energy = -s0[:, 4].cpu().numpy()
+    assert np.all(energy[status == 2] > 0.999)
+    d0 = geodesics._diagnostics(st, s0, ("ham", "lz"))
+    d1 = geodesics._diagnostics(st, s1, ("ham", "lz"))
+    ok = torch.from_numpy(status == 6).cuda()
+    dlz = ((d1["lz"] - d0["lz"]).abs() / d0["lz"].abs())[ok]
+    dh = (d1["ham"] - d0["ham"]).abs()[ok]
+    steps = int(sum(int(r.steps.sum().item()) for r in res))
+    print(f"orbits: {steps} steps, batched {t_batched * 1e3:.0f} ms (wall, one launch), 16 plain "
+          f"launches {t_plain * 1e3:.0f} ms; statuses {counts.tolist()}; L_z drift median "
+          f"{float(dlz.median()):.2e} max {float(dlz.max()):.2e}; H drift median "
+          f"{float(dh.median()):.2e} max {float(dh.max()):.2e}")
+    assert float(dlz.median()) < 1e-10 and float(dlz.max()) < 1e-5
+    assert float(dh.median()) < 1e-11 and float(dh.max()) < 1e-6
+    g = golden("c5_orbits.npz")
+    k = 0
+    all_final = s1.cpu().numpy()
+    all_steps = torch.cat([r.steps for r in res]).cpu().numpy()
+    all_s0 = s0.cpu().numpy()
+    while f"o{k}_index" in g.files:
+        i = int(g[f"o{k}_index"])
+        np.testing.assert_array_equal(all_s0[i:i + 1], g[f"o{k}_state0"])
+        assert_trace_parity(all_final[i:i + 1], status[i:i + 1], all_steps[i:i + 1],
+                            g[f"o{k}_final"], np.array([6]), np.array([int(g[f"o{k}_steps"])]),
+                            label=f"orbit {i}", spread=np.array([float(g[f"o{k}_spread"])]),
+                            spread_status=(6,))
+        k += 1
+    assert k >= 6
+
+
+def test_timelike_orbit_workload_adjoint_full_size(hip):
+    """Gradients of the final positions of all 65,536 workload orbits with
+    respect to the spin and every initial state (K = 32): finite for every
+    orbit that reaches tau = 1e4; per orbit bitwise equal to a launch of a
+    few of those orbits alone; and for sampled orbits equal to central
+    differences (h = 1e-6 in p_x, p_y) of the final position to 1e-5."""
+    import torch
+
+    from blackhorizon_amd import adjoint, geodesics, kerr
+
+    jobs = workloads.orbit_batches()
+    st = kerr.KerrSpacetime(1.0, 0.9)
+    pos = torch.from_numpy(np.concatenate([j.positions for j in jobs])).cuda()
+    vel = torch.from_numpy(np.concatenate([j.velocities for j in jobs])).cuda()
+    s0 = geodesics.build_state(pos, geodesics.timelike_momentum(st, pos, vel))
+    cfg = adjoint.TraceSettings(escape_radius=1000.0, max_steps=10_000_000, rtol=1e-10,
+                                atol=1e-12, initial_step=0.05, min_step=1e-14, K=32,
+                                lambda_end=1.0e4)
+    w = torch.randn((s0.shape[0], 3), dtype=torch.float64, device="cuda",
+                    generator=torch.Generator(device="cuda").manual_seed(3))
+    sg = s0.clone().requires_grad_(True)
+    a = torch.tensor(0.9, dtype=torch.float64, device="cuda", requires_grad=True)
+    Y, status = adjoint.trace_terminal(sg, a, cfg)
+    (Y[:, 1:4] * w).sum().backward()
+    ok = status == 6
+    assert int(ok.sum()) >= 0.97 * s0.shape[0]
+    assert bool(torch.isfinite(sg.grad[ok]).all()) and math.isfinite(float(a.grad))
+    pick = torch.tensor([22919, 14205, 8749, 46395, 100, 40000], device="cuda")
+    assert bool(ok[pick].all())
+    sp = s0[pick].clone().requires_grad_(True)
+    Yp, _ = adjoint.trace_terminal(sp, 0.9, cfg)
+    (Yp[:, 1:4] * w[pick]).sum().backward()
+    assert torch.equal(sp.grad, sg.grad[pick])
+    h = 1e-6
+    for i in range(3):
+        for j in (5, 6):
+            d = torch.zeros_like(s0[pick[i:i + 1]])
+            d[0, j] = h
+            with torch.no_grad():
+                rp = adjoint.Recorded(s0[pick[i:i + 1]] + d, 0.9, cfg)
+                rm = adjoint.Recorded(s0[pick[i:i + 1]] - d, 0.9, cfg)
+            assert int(rp.steps[0]) == int(rm.steps[0])
+            fd = float(((rp.terminal()[0, 1:4] - rm.terminal()[0, 1:4]) * w[pick[i]]).sum()) / (
+                2 * h)
+            g = float(sp.grad[i, j])
+            print(f"orbit {int(pick[i])} d/dp_{j - 4}: fd {fd:.10g} adjoint {g:.10g}")
+            assert abs(fd - g) <= 1e-5 * abs(fd) + 1e-8, (int(pick[i]), j, fd, g)

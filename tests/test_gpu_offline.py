@@ -29,17 +29,30 @@ def _settings(**kw):
 
 
 def _oracle_spread_status(st, s0, kw):
-    """Rays whose oracle status changes under ulp-level step-factor changes."""
+    """Rays whose oracle status changes under ulp-level step-factor changes,
+    and every status the oracle gave each ray under those changes."""
     ref = off.trace_tile(st, s0, **kw)
     unstable = np.zeros(len(s0), bool)
+    seen = [ref.status]
     for fn in (orc.step_factor_libm, orc.step_factor_ulp_noise(1)):
         alt = off.trace_tile(st, s0, factor_fn=fn, **kw)
         unstable |= alt.status != ref.status
-    return ref, unstable
+        seen.append(alt.status)
+    return ref, unstable, np.stack(seen, 1)
 
 
-def _compare(got, ref, unstable, tol):
+def _compare(got, ref, unstable, tol, seen=None, label=""):
+    """Statuses identical on every stable ray; on each unstable ray (whose
+    reference status itself flips under ulp-level step-factor changes) the
+    device status must be one of the statuses the reference gave it.  The
+    unstable rays are counted, printed and capped at 2% of the frame."""
     same = ~unstable
+    n_unstable = int(unstable.sum())
+    print(f"[offline parity] {label}: {len(unstable)} rays, {n_unstable} with a status the "
+          f"reference's own step-factor rounding flips ({100.0 * n_unstable / len(unstable):.2f}%)")
+    assert unstable.mean() < 0.02
+    if seen is not None:
+        assert np.all((seen[unstable] == got.status[unstable][:, None]).any(axis=1))
     np.testing.assert_array_equal(got.status[same], ref.status[same])
     ok = same & (got.status == ref.status)
     hit = ok & (ref.status == 5)
@@ -53,18 +66,22 @@ def _compare(got, ref, unstable, tol):
     return ok
 
 
-@pytest.mark.parametrize("mode,tol", [("precise", 1e-9), ("fast", 1e-6)])
-def test_trace_tile_matches_oracle(hip, mode, tol):
+@pytest.mark.parametrize("mode,rtol,tol", [("precise", 1e-9, 1e-9), ("fast", 1e-8, 1e-7),
+                                           ("fast", 1e-9, 1e-6)])
+def test_trace_tile_matches_oracle(hip, mode, rtol, tol):
+    """Disk hits and escape directions: the precise build to 1e-9 at the
+    renderer's rtol 1e-9; the fast build to the 1e-7 bar at rtol 1e-8, where
+    "auto" uses it, and to 1e-6 at rtol 1e-9, beyond that range (its
+    last-ulp rounding moves step sizes by ~1e-16 / rtol)."""
     from blackhorizon_amd import offline
 
     st, s0 = _frame()
     inner = st.isco_radius(True)
-    kw = dict(escape_radius=1.3 * 26.0, disk_radii=(inner, 18.0), rtol=1e-9, max_steps=3000)
-    ref, unstable = _oracle_spread_status(st, s0, kw)
-    assert unstable.mean() < 0.02
+    kw = dict(escape_radius=1.3 * 26.0, disk_radii=(inner, 18.0), rtol=rtol, max_steps=3000)
+    ref, unstable, seen = _oracle_spread_status(st, s0, kw)
     got = offline.trace_tile(hip.KerrSpacetime(spin=0.9), s0, kw["escape_radius"], _settings(),
                              kw["disk_radii"], kw["rtol"], kw["max_steps"], mode=mode)
-    ok = _compare(got, ref, unstable, tol)
+    ok = _compare(got, ref, unstable, tol, seen, f"{mode} rtol {rtol:g}")
     if mode == "precise":
         np.testing.assert_array_equal(got.steps[ok & (ref.status != 5)],
                                       ref.steps[ok & (ref.status != 5)])

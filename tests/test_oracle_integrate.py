@@ -41,3 +41,26 @@ def test_golden_finals_conserve_invariants():
     np.testing.assert_allclose(st.kerr_schild_radius(s1[:, 1], s1[:, 2], s1[:, 3]), 6.0,
                                rtol=1e-6)
     assert abs(s1[0, 3]) < 1e-8
+
+
+def test_integrate_dense_matches_reference_on_workload_orbits():
+    """Two of the timelike-orbit workload's orbits (c5_orbits.npz: proper
+    time 1e4 M, rtol 1e-10), bit for bit; all eight golden orbits conserve
+    E exactly, L_z to 1e-9 and H to 1e-10 over 5,000-7,000 steps."""
+    st = orc.Kerr(spin=0.9)
+    g = golden("c5_orbits.npz")
+    for k in (0, 3):
+        states = orc.integrate_dense(lambda y: orc.geodesic_rhs(st, y), g[f"o{k}_state0"],
+                                     1.0e4, rtol=1e-10)
+        assert len(states) - 1 == int(g[f"o{k}_steps"])
+        np.testing.assert_array_equal(states[-1], g[f"o{k}_final"])
+    k = 0
+    while f"o{k}_index" in g.files:
+        s0, s1 = g[f"o{k}_state0"], g[f"o{k}_final"]
+        e0, lz0 = orc.conserved_quantities(s0)
+        e1, lz1 = orc.conserved_quantities(s1)
+        np.testing.assert_array_equal(e1, e0)
+        np.testing.assert_allclose(lz1, lz0, rtol=1e-9)
+        np.testing.assert_allclose(orc.hamiltonian(st, s1), orc.hamiltonian(st, s0), atol=1e-10)
+        k += 1
+    assert k == 8
