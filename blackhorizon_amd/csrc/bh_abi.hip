@@ -4,6 +4,7 @@
 #include <string.h>
 
 #include <cmath>
+#include <mutex>
 
 #include "bh_internal.h"
 
@@ -207,6 +208,69 @@ int32_t bh_trace_batched_f64(const bh_trace_params *jobs, const int64_t *ray_off
 // must fit int32).
 constexpr int32_t MAX_ADJOINT_K = 1 << 20;
 
+// Argument checks shared by the two reverse-pass entries.
+static int32_t check_adjoint_args(const bh_trace_params *p, int64_t n, int32_t K,
+                                  const double *ckpt, const int32_t *event,
+                                  const int64_t *ev_step, const double *evy1, const double *prev,
+                                  const double *theta, const double *gterm, double *gy0,
+                                  double *gspin) {
+  int32_t rc = check_params(p);
+  if (rc) return rc;
+  if (!rays_ok(n)) return fail(BH_EINVAL, "bad n");
+  if (K < 1 || K > MAX_ADJOINT_K) return fail(BH_EINVAL, "need 1 <= K <= 2^20");
+  if (n > 0 && (!ckpt || !event || !ev_step || !evy1 || !prev || !theta || !gterm || !gy0 ||
+                !gspin))
+    return fail(BH_EINVAL, "NULL buffer");
+  return BH_OK;
+}
+
+static int32_t launch_adjoint(const bh_trace_params *p, int64_t n, int32_t K, double score_radius,
+                              const double *ckpt, const int32_t *event, const int64_t *ev_step,
+                              const double *evy1, const double *prev, const double *theta,
+                              const double *gterm, double *gy0, double *gspin, double *scratch,
+                              const int64_t *order, void *stream, const char *name) {
+  JobConst c = make_job_const(*p);
+  AdjointIO a{ckpt,  event, ev_step, evy1,  prev,         theta, gterm, gy0,
+              gspin, scratch, order, score_radius, n, K};
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = p->mode == BH_MODE_PRECISE ? precise::launch_trace_adjoint(c, a, s)
+                                            : fast::launch_trace_adjoint(c, a, s);
+  return hip_status(e, name);
+}
+
+// Scratch of bh_trace_adjoint_static_f64: one copy of this array lives on
+// every GPU the library runs on, from the first use there until the process
+// ends.  In host code its name is only a stand-in for the symbol, never a
+// device address: the device address is resolved once per device with
+// hipGetSymbolAddress (that device current) and cached, and only that
+// address reaches kernels.  Calls that overlap on one GPU share the array
+// and race; the backward writes every slot it reads in the same call, so
+// nothing needs clearing between calls.
+static __device__ __align__(256) double scratch[BH_ADJOINT_STATIC_SCRATCH_DOUBLES];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mutex;
+static double *g_scratch_ptr[MAX_DEVICES];
+
+static int32_t static_scratch(double **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_status(e, "bh_trace_adjoint_static_f64: hipGetDevice");
+  if (dev < 0 || dev >= MAX_DEVICES)
+    return fail(BH_EINVAL, "bh_trace_adjoint_static_f64: device ordinal beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess)
+      return hip_status(e, "bh_trace_adjoint_static_f64: hipGetSymbolAddress(scratch)");
+    if (!ptr) return fail(BH_EHIP, "bh_trace_adjoint_static_f64: scratch has no device address");
+    g_scratch_ptr[dev] = (double *)ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return BH_OK;
+}
+
 int32_t bh_adjoint_scratch_doubles(int32_t K) {
   return K < 1 || K > MAX_ADJOINT_K ? -1 : (int32_t)adjoint_scratch_per_ray(K);
 }
@@ -243,20 +307,36 @@ int32_t bh_trace_adjoint_f64(const bh_trace_params *p, int64_t n, int32_t K, dou
                              const double *evy1, const double *prev, const double *theta,
                              const double *gterm, double *gy0, double *gspin, double *scratch,
                              const int64_t *order, void *stream) {
-  int32_t rc = check_params(p);
+  int32_t rc = check_adjoint_args(p, n, K, ckpt, event, ev_step, evy1, prev, theta, gterm, gy0,
+                                  gspin);
   if (rc) return rc;
-  if (!rays_ok(n)) return fail(BH_EINVAL, "bad n");
-  if (K < 1 || K > MAX_ADJOINT_K) return fail(BH_EINVAL, "need 1 <= K <= 2^20");
-  if (n > 0 && (!ckpt || !event || !ev_step || !evy1 || !prev || !theta || !gterm || !gy0 ||
-                !gspin || !scratch))
-    return fail(BH_EINVAL, "NULL buffer");
-  JobConst c = make_job_const(*p);
-  AdjointIO a{ckpt,  event, ev_step, evy1,  prev,         theta, gterm, gy0,
-              gspin, scratch, order, score_radius, n, K};
-  hipStream_t s = (hipStream_t)stream;
-  hipError_t e = p->mode == BH_MODE_PRECISE ? precise::launch_trace_adjoint(c, a, s)
-                                            : fast::launch_trace_adjoint(c, a, s);
-  return hip_status(e, "bh_trace_adjoint_f64");
+  if (n > 0 && !scratch) return fail(BH_EINVAL, "NULL buffer");
+  return launch_adjoint(p, n, K, score_radius, ckpt, event, ev_step, evy1, prev, theta, gterm,
+                        gy0, gspin, scratch, order, stream, "bh_trace_adjoint_f64");
+}
+
+int32_t bh_trace_adjoint_static_f64(const bh_trace_params *p, int64_t n, int32_t K,
+                                    double score_radius, const double *ckpt, const int32_t *event,
+                                    const int64_t *ev_step, const double *evy1,
+                                    const double *prev, const double *theta, const double *gterm,
+                                    double *gy0, double *gspin, const int64_t *order,
+                                    void *stream) {
+  int32_t rc = check_adjoint_args(p, n, K, ckpt, event, ev_step, evy1, prev, theta, gterm, gy0,
+                                  gspin);
+  if (rc) return rc;
+  if (n == 0) return BH_OK;
+  // per wavefront of 64 rays, 9 K doubles per ray (n <= 2^36, K <= 2^20: no overflow)
+  const int64_t need = adjoint_scratch_per_ray(K) * ((n + 63) / 64) * 64;
+  if (need > BH_ADJOINT_STATIC_SCRATCH_DOUBLES)
+    return fail(BH_EINVAL,
+                "bh_trace_adjoint_static_f64: 9 K doubles per ray (n rounded up to a multiple "
+                "of 64) exceed the static scratch of BH_ADJOINT_STATIC_SCRATCH_DOUBLES (2^23); "
+                "use bh_trace_adjoint_f64 with a caller scratch buffer");
+  double *ptr = nullptr;
+  rc = static_scratch(&ptr);
+  if (rc) return rc;
+  return launch_adjoint(p, n, K, score_radius, ckpt, event, ev_step, evy1, prev, theta, gterm,
+                        gy0, gspin, ptr, order, stream, "bh_trace_adjoint_static_f64");
 }
 
 int32_t bh_rhs_vjp_f64(double mass, double spin, const double *state, const double *v,

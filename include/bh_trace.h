@@ -194,6 +194,30 @@ int32_t bh_trace_adjoint_f64(const bh_trace_params *p, int64_t n, int32_t K, dou
                              const double *gterm, double *gy0, double *gspin, double *scratch,
                              const int64_t *order, void *stream);
 
+/* Doubles in the static scratch of bh_trace_adjoint_static_f64 (64 MiB per
+ * GPU). */
+#define BH_ADJOINT_STATIC_SCRATCH_DOUBLES (8 * 1024 * 1024)
+
+/* bh_trace_adjoint_f64 without the scratch argument: allocates nothing (no
+ * hipMalloc, hipFree or allocating library routine), for callers that
+ * cannot allocate -- code under HIP graph capture, tight fitting or serving
+ * loops, harnesses that forbid allocation.  No counterpart in the
+ * reference.  The scratch is a file-scope device array of
+ * BH_ADJOINT_STATIC_SCRATCH_DOUBLES doubles, one per GPU, held until the
+ * process ends; its device address is resolved on the first call on each
+ * device (make that first call outside graph capture).  Limit: 9 K doubles
+ * per ray with n rounded up to a multiple of 64 must fit
+ * (9 K ceil(n / 64) 64 <= 2^23, e.g. n <= 29,120 rays at K = 32); larger
+ * calls return BH_EINVAL.  Calls that overlap on one GPU share the array
+ * and race: serialise them (one stream, or events).  Results equal
+ * bh_trace_adjoint_f64's bit for bit. */
+int32_t bh_trace_adjoint_static_f64(const bh_trace_params *p, int64_t n, int32_t K,
+                                    double score_radius, const double *ckpt,
+                                    const int32_t *event, const int64_t *ev_step,
+                                    const double *evy1, const double *prev,
+                                    const double *theta, const double *gterm, double *gy0,
+                                    double *gspin, const int64_t *order, void *stream);
+
 /* Vector-Jacobian product of geodesic_rhs (geodesics.py:42-74) at n states
  * (rows): zbar = J^T v (n, 8) and abar = v . d f / d spin (n). */
 int32_t bh_rhs_vjp_f64(double mass, double spin, const double *state, const double *v,

@@ -135,6 +135,26 @@ std::vector<Case> build_cases() {
     p.max_steps = INT64_MIN;
     return bh_count_outside_f64(&p, d, 4, 4, i32, nullptr);
   }, "max_steps out of range");
+  add("adjoint static: beyond the static scratch", [=] {
+    bh_trace_params p = good_params();
+    return bh_trace_adjoint_static_f64(&p, 29121, 32, 4.0, d, i32, i64, d, d, d, d, d, d,
+                                       nullptr, nullptr);
+  }, "static scratch");
+  add("adjoint static: INT64_MAX rays", [=] {
+    bh_trace_params p = good_params();
+    return bh_trace_adjoint_static_f64(&p, BIG, 32, 4.0, d, i32, i64, d, d, d, d, d, d,
+                                       nullptr, nullptr);
+  }, "bad n");
+  add("adjoint static: K 2^21", [=] {
+    bh_trace_params p = good_params();
+    return bh_trace_adjoint_static_f64(&p, 64, 1 << 21, 4.0, d, i32, i64, d, d, d, d, d, d,
+                                       nullptr, nullptr);
+  }, "K <= 2^20");
+  add("adjoint static: max rays at K 2^20", [=] {
+    bh_trace_params p = good_params();
+    return bh_trace_adjoint_static_f64(&p, (int64_t)1 << 36, 1 << 20, 4.0, d, i32, i64, d, d, d,
+                                       d, d, d, nullptr, nullptr);
+  }, "static scratch");
   add("plain: max_steps -2^61", [=] {
     bh_trace_params p = good_params();
     p.max_steps = -((int64_t)1 << 61);

@@ -164,3 +164,28 @@ def test_controller_and_colour_entries_reject_bad_arguments():
     assert lib.bh_error_ratio_f64(0, None, None, None, 0, 2, 1e-8, 1e-10, None, None) == 0
     assert lib.bh_step_factor_f64(1, None, 0, None, None) == 0
     assert lib.bh_checker_colors_u8(None, 0, 12, None, None) == 0
+
+
+def test_static_scratch_adjoint_refuses_beyond_its_limit():
+    """The allocation-free reverse pass refuses, before touching a device,
+    calls whose scratch (9 K doubles per ray, rays rounded up to 64) exceeds
+    its static array, and the arguments the allocating entry refuses."""
+    lib = _lib.load()
+    buf = (ctypes.c_double * 64)()
+    p = ctypes.cast(buf, ctypes.c_void_p)
+    limit = 8 * 1024 * 1024
+    args = lambda n, K: (_params(), n, K, 4.0, p, p, p, p, p, p, p, p, p, None, None)  # noqa
+    # 29,120 rays at K = 32 fit (9 * 32 * 29120 <= 2^23); 29,184 (the next multiple of 64) do not
+    assert 9 * 32 * 29120 <= limit < 9 * 32 * 29184
+    rc = lib.bh_trace_adjoint_static_f64(*args(29121, 32))
+    assert rc == _lib.BH_EINVAL
+    assert "static scratch" in lib.bh_last_error().decode()
+    rc = lib.bh_trace_adjoint_static_f64(*args(1 << 20, 1))
+    assert rc == _lib.BH_EINVAL and "static scratch" in lib.bh_last_error().decode()
+    assert lib.bh_trace_adjoint_static_f64(*args(64, 0)) == _lib.BH_EINVAL
+    assert "K <= 2^20" in lib.bh_last_error().decode()
+    rc = lib.bh_trace_adjoint_static_f64(_params(), 64, 8, 4.0, None, p, p, p, p, p, p, p, p,
+                                         None, None)
+    assert rc == _lib.BH_EINVAL and "NULL" in lib.bh_last_error().decode()
+    # nothing to reverse: no device work at all
+    assert lib.bh_trace_adjoint_static_f64(*args(0, 8)) == _lib.BH_OK
