@@ -20,7 +20,15 @@ against the FP64 vector peak (78.6 TFLOP/s).  cpu_baseline: the NumPy oracle
 (a restatement of the reference tracer) on a seeded subsample of every frame,
 on this host's cores, timed before the GPU is touched.  adjoint_over_forward:
 the discrete-adjoint op's time over the plain forward trace's on the 512^2
-shadow fit (--workload shadowfit), measured after the sweep.
+shadow fit (--workload shadowfit), measured after the sweep.  Also after the
+sweep (``targets``; --no-targets skips them): batched_over_per_job, the
+uneven 64-job mix as one persistent batched plan against one plain launch
+per job, both autotuned (--workload uneven); the 65,536-orbit timelike
+workload's forward time, E / L_z / H drift and adjoint/forward (--workload
+orbits); and one a = 0.9 frame's ray generation (host directions, device
+states) and host<->device copies, reported apart from the kernel time.
+Launch times are reported as means (roofline, matching rocprof's average)
+and medians.
 """
 
 from __future__ import annotations
@@ -203,6 +211,75 @@ def pick_tune(args, prepared, kind_key):
     return res.best, info
 
 
+def frame_io(job, mode: str, tune, reps: int = 5) -> dict:
+    """One 1920x1080 frame's costs outside the tracer kernel, each the median
+    of ``reps`` runs: camera directions on the host (NumPy, as the reference
+    builds them, camera.py:96-113), the whole state built on the device
+    (camera_rays_kernel), the (n, 8) initial states copied host -> device
+    from pageable and from pinned memory, and the results (states, status,
+    steps) copied device -> host; plus the frame's trace kernel time and the
+    ray-step rate with both copies included."""
+    import torch
+
+    from blackhorizon_amd import kerr, tracer
+    from blackhorizon_amd.camera import PinholeCamera
+
+    st = kerr.KerrSpacetime(job.mass, job.spin)
+    cam = PinholeCamera.from_orbit(job.distance, job.inclination_degrees,
+                                   azimuth_degrees=job.azimuth_degrees,
+                                   fov_degrees=job.fov_degrees, width=job.width,
+                                   height=job.height)
+    cur = torch.cuda.current_stream()
+
+    def dev_ms(fn):
+        out = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record(cur)
+            fn()
+            e1.record(cur)
+            torch.cuda.synchronize()
+            out.append(e0.elapsed_time(e1))
+        return float(np.median(out))
+
+    def host_ms(fn):
+        out = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            out.append((time.perf_counter() - t0) * 1e3)
+        return float(np.median(out))
+
+    ms_dirs_host = host_ms(cam.ray_directions)
+    holder = {}
+    ms_states_dev = dev_ms(lambda: holder.__setitem__("s", cam.device_states(st)))
+    s_dev = holder["s"]
+    s_host = s_dev.cpu().numpy()
+    pinned = torch.from_numpy(s_host).pin_memory()
+    ms_h2d = dev_ms(lambda: torch.from_numpy(s_host).to("cuda"))
+    ms_h2d_pinned = dev_ms(lambda: pinned.to("cuda", non_blocking=True))
+    plan = tracer.TracePlan(st, s_dev, job.escape_radius, job.max_steps, job.rtol, job.atol,
+                            job.initial_step, job.max_step, job.min_step, job.capture_margin,
+                            mode=mode, ray_order=torch.from_numpy(cam.center_tile_order(8)))
+    plan.launch(tune)
+    ms_kernel = dev_ms(lambda: plan.launch(tune))
+    res = plan.result()  # device tensors, like the state it traced
+
+    def d2h():
+        res.states.cpu()
+        res.status.cpu()
+        res.steps.cpu()
+    ms_d2h = dev_ms(d2h)
+    steps = int(plan.steps.sum().item())
+    return dict(spin=job.spin, inclination=job.inclination_degrees, rays=job.n_rays,
+                ms_camera_directions_host=ms_dirs_host, ms_states_on_device=ms_states_dev,
+                ms_h2d_states_pageable=ms_h2d, ms_h2d_states_pinned=ms_h2d_pinned,
+                ms_d2h_results=ms_d2h, ms_trace_kernel=ms_kernel,
+                ray_steps=steps, ray_steps_per_s_kernel=steps / (ms_kernel * 1e-3),
+                ray_steps_per_s_with_copies=steps / ((ms_h2d + ms_kernel + ms_d2h) * 1e-3))
+
+
 def run_uneven(args) -> dict:
     """The uneven 64-job mix: one persistent batched launch per arithmetic
     mode (fast jobs and precise jobs on two streams) against one plain
@@ -229,7 +306,8 @@ def run_uneven(args) -> dict:
     cur = torch.cuda.current_stream()
     tune_info = None
     if args.tune == "auto":
-        res = autotune.autotune(plan, workload_key="uneven64", cache=None, reps=2)
+        res = autotune.autotune(plan, workload_key="uneven64", cache=None,
+                                reps=getattr(args, "tune_reps", 2))
         tune_info = dict(best=res.best.__dict__, best_ms=res.best_ms, candidates=len(res.table))
         print(f"[bench] autotune batched: {res.best}", file=sys.stderr, flush=True)
     elif args.tune != "default":
@@ -260,7 +338,8 @@ def run_uneven(args) -> dict:
     per_job_tune_info = None
     if args.tune == "auto":
         # the baseline gets its own best launch shape too
-        table = autotune.time_candidates(per_job, autotune.plain_candidates(), reps=2)
+        table = autotune.time_candidates(per_job, autotune.plain_candidates(),
+                                         reps=getattr(args, "tune_reps", 2))
         best, best_ms = autotune.select(table)
         per_job.tune = best
         per_job_tune_info = dict(best=best.__dict__, best_ms=best_ms, candidates=len(table))
@@ -562,7 +641,7 @@ def run_shadowfit(args) -> dict:
         print(f"[bench] shadowfit pass {k} done", file=sys.stderr, flush=True)
     med = {k: float(np.median(v)) for k, v in times.items()}
     steps = int(res.steps.sum().item())
-    return dict(rays=size * size, ray_steps=steps, loss=float(loss),
+    return dict(rays=size * size, ray_steps=steps, loss=float(loss.detach()),
                 grad_spin=float(g[0]), grad_inclination=float(g[1]), grad_distance=float(g[2]),
                 ms_plain_forward=med["plain"], ms_recording_forward=med["record"],
                 ms_adjoint=med["adjoint"], ms_fit_step=med["fit_step"],
@@ -731,6 +810,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="hd/hd09: skip the adjoint/forward measurement (512^2 shadow fit)")
+    ap.add_argument("--no-targets", action="store_true",
+                    help="hd: skip the uneven-mix, timelike-orbit and frame-copy measurements")
     args = ap.parse_args(argv)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -825,12 +906,42 @@ def main(argv=None) -> int:
                               rays=job.n_rays, ray_steps=s, attempts=a,
                               iterations=int(plan.iters.item()),
                               kernel_ms=round(float(np.mean(ms)), 3),
+                              kernel_ms_median=round(float(np.median(ms)), 4),
                               ns_per_ray_step=float(np.mean(ms)) * 1e6 / max(s, 1),
                               pct_fp64_peak=100.0 * a * FLOP_PER_ATTEMPT
                               / (float(np.mean(ms)) * 1e-3) / (FP64_VECTOR_PEAK_TFLOPS * 1e12),
                               status_counts=st[:7]))
         steps_sweep += s
         attempts_sweep += a
+    targets = None
+    if args.workload == "hd" and not args.no_targets:
+        # the north star's other targets, outside the timed sweep: the
+        # uneven 64-job mix (one persistent batched launch vs one plain
+        # launch per job, both autotuned), the 65,536-orbit timelike
+        # workload, and one frame's ray-generation and copy costs
+        io_job = next(j for j in jobs if j.spin == 0.9)
+        del prepared
+        torch.cuda.empty_cache()
+        targets = {"frame_io": frame_io(io_job, args.mode, tune)}
+        print("[bench] frame ray generation and copies measured", file=sys.stderr, flush=True)
+        sub = argparse.Namespace(**{**vars(args), "steps": 3, "warmup": 1, "tune_reps": 1})
+        un = run_uneven(sub)
+        targets["uneven"] = {k: un[k] for k in (
+            "speedup", "ms_batched", "ms_per_job", "speedup_graph", "ms_batched_graph",
+            "ms_per_job_graph", "batched_ray_steps_per_s", "per_job_ray_steps_per_s",
+            "pct_fp64_vector_peak_batched", "rays", "ray_steps", "per_ray_steps_identical",
+            "jobs_per_mode", "tune", "tune_per_job")}
+        print("[bench] uneven mix measured", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+        orb = run_orbits(argparse.Namespace(**{**vars(args), "steps": 3, "warmup": 1,
+                                               "orbits": 65536, "proper_time": 1.0e4}))
+        targets["orbits"] = {k: orb[k] for k in (
+            "orbits", "jobs", "proper_time", "ray_steps", "ms_forward", "ray_steps_per_s",
+            "status_counts", "energy_drift_max", "lz_rel_drift_max", "lz_rel_drift_median",
+            "ham_drift_max", "ham_drift_median", "steps_per_orbit", "ms_recording_forward",
+            "ms_adjoint", "adjoint_over_forward", "grad_finite")}
+        print("[bench] timelike orbits measured", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
     adj = None
     if not args.no_adjoint and args.workload in ("hd", "hd09"):
         # the metric's third part: adjoint op time over plain forward time on
@@ -839,6 +950,7 @@ def main(argv=None) -> int:
         adj = run_shadowfit(fit_args)
         print("[bench] adjoint/forward measured", file=sys.stderr, flush=True)
     total_kernel_s = sum(kern_ms) * 1e-3
+    sweep_median_ms = float(sum(f["kernel_ms_median"] for f in per_frame))
     value = steps_sweep * args.steps / wall
     flop = attempts_sweep * args.steps * FLOP_PER_ATTEMPT
     achieved_tflops = flop / total_kernel_s / 1e12
@@ -874,6 +986,13 @@ def main(argv=None) -> int:
         "attempts_per_s": attempts_sweep * args.steps / wall,
         "pct_fp64_vector_peak": 100.0 * achieved_tflops / FP64_VECTOR_PEAK_TFLOPS,
         "adjoint_over_forward": None if adj is None else adj["adjoint_over_forward"],
+        "batched_over_per_job": None if targets is None else targets["uneven"]["speedup"],
+        "orbits_forward_ms": None if targets is None else targets["orbits"]["ms_forward"],
+        # per frame the median of its timed launches, summed over the sweep
+        "sweep_kernel_ms_median": sweep_median_ms,
+        "pct_fp64_vector_peak_median": 100.0 * attempts_sweep * FLOP_PER_ATTEMPT / (
+            sweep_median_ms * 1e-3) / (FP64_VECTOR_PEAK_TFLOPS * 1e12),
+        "targets": targets,
         "adjoint": None if adj is None else {
             k: adj[k] for k in ("rays", "ray_steps", "ms_plain_forward", "ms_recording_forward",
                                 "ms_adjoint", "ms_fit_step", "adjoint_over_forward")},
