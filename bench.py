@@ -782,6 +782,42 @@ def run_orbits(args) -> dict:
     return out
 
 
+def run_dense(args) -> dict:
+    """integrate_dense (integrators.py:127-174) on the device: --orbits
+    near-circular orbits around a = 0.9 outside r = 15 M (2% velocity
+    spread, so no orbit plunges and sinks the shared step), one batch-shared
+    adaptive step over proper time --dense-lambda at rtol 1e-10, precise
+    build, the whole trajectory kept on the device.  Every attempt runs all
+    rays, so FLOP = attempts x rays x 1321."""
+    import torch
+
+    from blackhorizon_amd import _lib, geodesics, integrators, kerr, workloads
+
+    _lib.require_device()
+    st = kerr.KerrSpacetime(1.0, 0.9)
+    p, v = workloads.near_circular_orbits(args.orbits, seed=3, r_min=15.0, spread=0.02)
+    pos = torch.from_numpy(p).cuda()
+    s0 = geodesics.build_state(pos, geodesics.timelike_momentum(st, pos,
+                                                                torch.from_numpy(v).cuda()))
+    g = integrators.GeodesicRHS(st, mode="precise")
+    times = []
+    traj = None
+    for k in range(args.warmup + args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        traj = integrators.integrate_dense(g, s0, args.dense_lambda, rtol=1e-10, chunk=1024)
+        torch.cuda.synchronize()
+        if k >= args.warmup:
+            times.append(time.perf_counter() - t0)
+        print(f"[bench] dense pass {k} done", file=sys.stderr, flush=True)
+    n = s0.shape[0]
+    accepted = len(traj) - 1
+    ms = float(np.median(times)) * 1e3
+    return dict(orbits=n, proper_time=args.dense_lambda, accepted_steps=accepted,
+                ray_steps=accepted * n, ms_median=ms, ray_steps_per_s=accepted * n / (ms * 1e-3),
+                us_per_accepted_step=ms * 1e3 / max(accepted, 1))
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -789,7 +825,9 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="hd",
                     choices=["hd", "hd09", "small", "uneven", "shadowfit", "orbits", "rk4",
-                             "offline", "shader"])
+                             "offline", "shader", "dense"])
+    ap.add_argument("--dense-lambda", type=float, default=100.0,
+                    help="dense: proper time of the batch-shared integration")
     ap.add_argument("--orbits", type=int, default=65536, help="orbits: number of orbits")
     ap.add_argument("--proper-time", type=float, default=1.0e4, help="orbits: proper time")
     ap.add_argument("--fit-size", type=int, default=512, help="shadowfit: image side")
@@ -821,6 +859,11 @@ def main(argv=None) -> int:
                                    "measured; run with --gpus 1"}))
         return 2
 
+    if args.workload == "dense":
+        res = run_dense(args)
+        print(json.dumps({"metric": "integrate_dense (batch-shared step) ray-steps/s",
+                          "workload": "dense", **res}), flush=True)
+        return 0
     if args.workload == "orbits":
         res = run_orbits(args)
         print(json.dumps({"metric": METRIC, "workload": "timelike orbits", "mode": args.mode,

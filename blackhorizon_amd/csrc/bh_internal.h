@@ -149,7 +149,7 @@ struct ShaderIO {
 struct DenseIO {
   double *y;            // [8][ld] in: states at the chunk's start; out: at its end
   int64_t ld;
-  double *work;         // [24][ld] scratch
+  double *work;         // 24 ld + 8192 scratch: SoA planes, then 2 planes of per-block maxima
   double *traj;         // [cap][8][n] accepted states of this chunk
   double *ctl;          // [2] elapsed affine parameter, step size (in/out)
   int64_t *count;       // [1] accepted states written

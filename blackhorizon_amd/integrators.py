@@ -211,7 +211,7 @@ def integrate_dense(rhs, state0: Array, lambda_total: float, rtol: float = 1e-10
     dev = s0.device
     y = torch.empty((8, n), dtype=torch.float64, device=dev)
     _lib.check(lib.bh_aos_to_soa_f64(_lib.ptr(s0), _lib.ptr(y), n, n, stream), "aos_to_soa")
-    work = torch.empty((24, n), dtype=torch.float64, device=dev)
+    work = torch.empty(24 * n + 8192, dtype=torch.float64, device=dev)
     traj = torch.empty((chunk, 8, n), dtype=torch.float64, device=dev)
     ctl = torch.tensor([0.0, float(initial_step)], dtype=torch.float64, device=dev)
     count = torch.zeros(1, dtype=torch.int64, device=dev)

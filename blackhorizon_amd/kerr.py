@@ -12,6 +12,7 @@ from __future__ import annotations
 import math
 from dataclasses import dataclass
 
+import numpy
 import torch
 
 from . import _dev, _lib
@@ -74,14 +75,21 @@ class KerrSpacetime:
 
     def geometry(self, x: Array, y: Array, z: Array, gradients: bool = False,
                  mode: str = "precise") -> KerrSchildGeometry:
-        """Kerr-Schild geometry at points (x, y, z) of shape (n,) (kerr.py:118-193)."""
-        ref = x
-        xyz = _dev.as_device_f64(
-            torch.stack([torch.as_tensor(v, dtype=torch.float64).reshape(-1).to(_dev.device())
-                         for v in (x, y, z)], dim=-1), (3,), "points")
+        """Kerr-Schild geometry at points (x, y, z), broadcastable arrays
+        (kerr.py:118-193): radius and h on the broadcast shape, l and grad_h
+        on shape + (3,), grad_l on shape + (3, 3).  The points are flattened
+        for the device and the results reshaped back; NumPy in -> NumPy out,
+        tensor in -> tensor out."""
+        ref = x if isinstance(x, (numpy.ndarray, torch.Tensor)) else (
+            y if isinstance(y, (numpy.ndarray, torch.Tensor)) else z)
+        dev = _dev.device()
+        comps = [v.detach().to(device=dev, dtype=torch.float64) if isinstance(v, torch.Tensor)
+                 else torch.as_tensor(numpy.asarray(v, dtype=numpy.float64)).to(dev)
+                 for v in (x, y, z)]
+        shape = torch.broadcast_shapes(*(c.shape for c in comps))
+        xyz = torch.stack([c.expand(shape).reshape(-1) for c in comps], dim=-1).contiguous()
         n = xyz.shape[0]
         lib = _lib.require_device()
-        dev = xyz.device
         radius = torch.empty(n, dtype=torch.float64, device=dev)
         h = torch.empty(n, dtype=torch.float64, device=dev)
         l = torch.empty((n, 3), dtype=torch.float64, device=dev)
@@ -90,30 +98,39 @@ class KerrSpacetime:
         _lib.check(lib.bh_geometry_f64(self.mass, self.spin, _mode(mode), _lib.ptr(xyz), n,
                                        _lib.ptr(radius), _lib.ptr(h), _lib.ptr(l), _lib.ptr(gh),
                                        _lib.ptr(gl), _lib.stream_handle()), "geometry")
-        out = [_dev.like_input(t, ref) if t is not None else None for t in (radius, h, l, gh, gl)]
+        tails = ((), (), (3,), (3,), (3, 3))
+        out = [None if t is None else _dev.like_input(t.reshape(tuple(shape) + tail), ref)
+               for t, tail in zip((radius, h, l, gh, gl), tails)]
         return KerrSchildGeometry(radius=out[0], h=out[1], l=out[2], grad_h=out[3], grad_l=out[4])
 
     def kerr_schild_radius(self, x: Array, y: Array, z: Array) -> Array:
-        """kerr.py:114-116."""
+        """kerr.py:114-116 (broadcastable arrays, as geometry)."""
         return self.geometry(x, y, z).radius
 
     def metric(self, positions: Array) -> Array:
-        """g_{mu nu} = eta + 2 H l_mu l_nu at (..., 4) positions (kerr.py:201-217)."""
+        """g_{mu nu} = eta + 2 H l_mu l_nu at positions of shape (..., 4)
+        (kerr.py:201-217); result (..., 4, 4)."""
         return self._metric(positions, inverse=False)
 
     def inverse_metric(self, positions: Array) -> Array:
-        """g^{mu nu} = eta - 2 H l^mu l^nu (kerr.py:219-235)."""
+        """g^{mu nu} = eta - 2 H l^mu l^nu at (..., 4) (kerr.py:219-235)."""
         return self._metric(positions, inverse=True)
 
     def _metric(self, positions: Array, inverse: bool) -> Array:
-        pos = _dev.as_device_f64(positions, (4,), "positions")
-        geo = self.geometry(pos[:, 1], pos[:, 2], pos[:, 3])
+        pos = positions.detach().to(device=_dev.device(), dtype=torch.float64) \
+            if isinstance(positions, torch.Tensor) else \
+            torch.as_tensor(numpy.asarray(positions, dtype=numpy.float64)).to(_dev.device())
+        if pos.dim() < 1 or pos.shape[-1] != 4:
+            raise ValueError(f"positions must have shape (..., 4), got {tuple(pos.shape)}")
+        lead = tuple(pos.shape[:-1])
+        flat = pos.reshape(-1, 4)
+        geo = self.geometry(flat[:, 1], flat[:, 2], flat[:, 3])
         l4 = torch.cat([(-1.0 if inverse else 1.0) * torch.ones_like(geo.h)[:, None], geo.l], -1)
         eta = torch.diag(torch.tensor([-1.0, 1.0, 1.0, 1.0], dtype=torch.float64,
-                                      device=pos.device)).expand(pos.shape[0], 4, 4)
+                                      device=pos.device)).expand(flat.shape[0], 4, 4)
         sign = -2.0 if inverse else 2.0
         g = eta + sign * geo.h[:, None, None] * (l4[:, :, None] * l4[:, None, :])
-        return _dev.like_input(g, positions)
+        return _dev.like_input(g.reshape(lead + (4, 4)), positions)
 
 
 def _mode(mode: str) -> int:

@@ -364,8 +364,10 @@ int32_t bh_midpoint_f64(double mass, double spin, int32_t mode, double *state_so
  * traj [cap][8][n]; *count (device int64) = states appended; *flag (device
  * int32) = BH_DENSE_DONE (elapsed >= lambda_total), BH_DENSE_CHUNK_FULL
  * (call again to continue) or BH_DENSE_UNDERFLOW (h < 1e-14: the
- * reference raises RuntimeError).  work: [24][ld] device doubles.  One
- * workgroup (meant for test and analysis batches, as the reference's). */
+ * reference raises RuntimeError).  work: 24 ld + 8192 device doubles.  One
+ * cooperative launch over the chip (a thread per ray up to the resident
+ * capacity, a grid-wide barrier per attempt); the trajectory does not
+ * depend on the grid size. */
 #define BH_DENSE_DONE 0
 #define BH_DENSE_CHUNK_FULL 1
 #define BH_DENSE_UNDERFLOW 2

@@ -543,3 +543,42 @@ def test_shadow_gpu_png_full_frame(hip):
     np.testing.assert_array_equal(esc, cls != 0)
     light = orc.checker_parity(orc.Kerr(spin=job.spin), res.states[esc])
     assert np.mean(light == (cls[esc] == 2)) > 0.999
+
+
+def test_geometry_and_metric_keep_the_callers_shapes(hip):
+    """kerr.py:118-135 and 201-235 take broadcastable (x, y, z) and (..., 4)
+    positions: a (4, 5) grid of x with a row of y and a scalar z, and a
+    (2, 3, 4) block of positions, give results on those shapes, bit for bit
+    the oracle's (precise build), as NumPy or as device tensors."""
+    import torch
+
+    rng = np.random.default_rng(12)
+    x = rng.uniform(-20.0, 20.0, (4, 5))
+    y = rng.uniform(-20.0, 20.0, 5)
+    z = 3.5
+    st = hip.KerrSpacetime(spin=0.8)
+    ost = orc.Kerr(spin=0.8)
+    got = st.geometry(x, y, z, gradients=True)
+    xb, yb, zb = np.broadcast_arrays(x, y, np.float64(z))
+    ref = ost.geometry(xb.ravel(), yb.ravel(), zb.ravel(), gradients=True)
+    for f, tail in (("radius", ()), ("h", ()), ("l", (3,)), ("grad_h", (3,)),
+                    ("grad_l", (3, 3))):
+        a = getattr(got, f)
+        assert a.shape == (4, 5) + tail, (f, a.shape)
+        np.testing.assert_array_equal(a.reshape((-1,) + tail), getattr(ref, f), err_msg=f)
+    r = st.kerr_schild_radius(torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda(),
+                              torch.tensor(z, dtype=torch.float64).cuda())
+    assert isinstance(r, torch.Tensor) and tuple(r.shape) == (4, 5)
+    np.testing.assert_array_equal(r.cpu().numpy(), got.radius)
+    pos = np.concatenate([np.zeros((2, 3, 1)), rng.uniform(-15.0, 15.0, (2, 3, 3))], -1)
+    g = st.metric(pos)
+    gi = st.inverse_metric(pos)
+    assert g.shape == (2, 3, 4, 4) and gi.shape == (2, 3, 4, 4)
+    flat = pos.reshape(-1, 4)
+    np.testing.assert_array_equal(g.reshape(-1, 4, 4), ost.metric(flat))
+    np.testing.assert_array_equal(gi.reshape(-1, 4, 4), ost.inverse_metric(flat))
+    np.testing.assert_allclose(np.einsum("...ij,...jk->...ik", g, gi),
+                               np.broadcast_to(np.eye(4), (2, 3, 4, 4)), atol=1e-12)
+    one = st.metric(pos[0, 0])
+    assert one.shape == (4, 4)
+    np.testing.assert_array_equal(one, g[0, 0])
