@@ -703,7 +703,9 @@ def run_shader(args) -> dict:
 def run_orbits(args) -> dict:
     """65,536 near-circular timelike orbits around a = 0.9 in 16 uneven jobs,
     integrated to proper time 1e4 M at rtol 1e-10 in one persistent batched
-    launch; E / L_z / H drift; adjoint of the final positions w.r.t. spin and
+    launch (--mode, "auto" = the precise build at this rtol); the same launch
+    in the fast build, with its largest final-state difference from the
+    first; E / L_z / H drift; adjoint of the final positions w.r.t. spin and
     initial momenta."""
     import torch
 
@@ -735,6 +737,22 @@ def run_orbits(args) -> dict:
     ms = e0.elapsed_time(e1) / args.steps
     print("[bench] orbits forward done", file=sys.stderr, flush=True)
     res = plan.results(as_numpy=False)
+    # the same orbits in the fast build: every attempt's latency is about
+    # half the precise build's; its final states are compared with the
+    # precise ones below
+    fplan = batch.BatchPlan(tjobs, mode="fast")
+    for _ in range(args.warmup):
+        fplan.launch()
+    torch.cuda.synchronize()
+    e0.record(cur)
+    for _ in range(args.steps):
+        fplan.launch()
+    e1.record(cur)
+    torch.cuda.synchronize()
+    ms_fast = e0.elapsed_time(e1) / args.steps
+    fres = fplan.results(as_numpy=False)
+    del fplan
+    print("[bench] orbits fast-build forward done", file=sys.stderr, flush=True)
     s0 = torch.cat(states)
     s1 = torch.cat([r.states for r in res])
     status = torch.cat([r.status for r in res])
@@ -744,10 +762,23 @@ def run_orbits(args) -> dict:
     d0 = geodesics._diagnostics(st, s0, ("ham", "energy", "lz"))
     d1 = geodesics._diagnostics(st, s1, ("ham", "energy", "lz"))
     ok = status == 6
+    f1 = torch.cat([r.states for r in fres])
+    fstatus = torch.cat([r.status for r in fres])
+    both = ok & (fstatus == 6)
+    # block-relative difference of the final states (t, position, p_t,
+    # momentum; each relative to max(norm, 1)), fast vs precise
+    fdiff = torch.zeros(int(both.sum().item()), dtype=torch.float64, device=s1.device)
+    for lo, hi in ((0, 1), (1, 4), (4, 5), (5, 8)):
+        a_, b_ = f1[both][:, lo:hi], s1[both][:, lo:hi]
+        fdiff = torch.maximum(fdiff, (a_ - b_).norm(dim=1) / b_.norm(dim=1).clamp(min=1.0))
     dlz = ((d1["lz"] - d0["lz"]).abs() / d0["lz"].abs())[ok]
     dh = (d1["ham"] - d0["ham"]).abs()[ok]
     out = dict(orbits=int(s0.shape[0]), jobs=len(jobs), proper_time=args.proper_time,
                ray_steps=steps, ms_forward=ms, ray_steps_per_s=steps / (ms * 1e-3),
+               ms_forward_fast_build=ms_fast,
+               fast_build_statuses_identical=bool(torch.equal(fstatus, status)),
+               fast_vs_precise_state_diff_max=float(fdiff.max()),
+               fast_vs_precise_state_diff_median=float(fdiff.median()),
                status_counts=torch.bincount(status.long(), minlength=7).tolist(),
                energy_drift_max=float((d1["energy"] - d0["energy"]).abs().max()),
                lz_rel_drift_max=float(dlz.max()), lz_rel_drift_median=float(dlz.median()),
@@ -980,6 +1011,8 @@ def main(argv=None) -> int:
                                                "orbits": 65536, "proper_time": 1.0e4}))
         targets["orbits"] = {k: orb[k] for k in (
             "orbits", "jobs", "proper_time", "ray_steps", "ms_forward", "ray_steps_per_s",
+            "ms_forward_fast_build", "fast_build_statuses_identical",
+            "fast_vs_precise_state_diff_max",
             "status_counts", "energy_drift_max", "lz_rel_drift_max", "lz_rel_drift_median",
             "ham_drift_max", "ham_drift_median", "steps_per_orbit", "ms_recording_forward",
             "ms_adjoint", "adjoint_over_forward", "grad_finite")}
