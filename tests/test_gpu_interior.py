@@ -8,10 +8,11 @@ Schwarzschild horizon, their whole sky traced backwards:
   (r = 1.9), the sky narrows deeper in but stays above 5% at r = 0.5, and
   escaping rays carry finite, moderate energies -- test_interior.py:64-103
   -- with every status equal to the oracle's bit for bit;
-* the offline tile tracer (render.py:184-309 with interior_stop): a 64 x 40
-  frame from r = 1.5 looking outward sees the outside universe (more than
-  5% of its pixels escape), is deterministic, and matches the offline
-  oracle -- test_offline.py:311-345, whose emission stage is out of scope.
+* the offline tile tracer (render.py:184-309 with interior_stop): 64 x 40
+  frames from r = 1.5 see the outside universe looking outward, a mix of sky
+  and stop radius looking sideways, the stop radius looking inward; they are
+  deterministic and match the offline oracle -- test_offline.py:311-345,
+  whose emission stage is out of scope.
 """
 
 import numpy as np
