@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define BH_ABI_VERSION 1
+#define BH_ABI_VERSION 2  /* 2: integrate_dense work is 24 ld + 8192 doubles; static-scratch adjoint */
 
 #define BH_OK 0
 #define BH_EINVAL -1   /* invalid argument (maps to ValueError) */
