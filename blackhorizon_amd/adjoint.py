@@ -326,7 +326,9 @@ def capture_margin(state0: torch.Tensor, spin, mass: float = 1.0, grid: int = 25
     with torch.no_grad():
         ad, lamd, etad = a.detach(), lam.detach(), eta.detach()
         lo_r = r_plus.detach().expand_as(r)
-        hi_r = r.detach()
+        # a state at or inside the horizon has no interval outside it: the
+        # search then collapses onto r_+
+        hi_r = torch.maximum(r.detach(), lo_r)
         u = torch.linspace(0.0, 1.0, grid, dtype=torch.float64, device=state0.device)
         rr = lo_r[:, None] + (hi_r - lo_r)[:, None] * u[None, :]
         vals = _radial_potential(rr, ad, mass, lamd[:, None], etad[:, None])
