@@ -204,6 +204,44 @@ def uneven_subsample(ref_trace, ref_kerr, n_jobs=8, per_job=48):
     np.savez_compressed(os.path.join(HERE, "uneven_subsample.npz"), **out)
 
 
+def uneven_all(ref_trace, ref_kerr, per_job=12):
+    """Seeded rays from every job of the 64-job uneven mix, traced by the
+    reference's trace_rays with each job's settings, plus ``spread``: per
+    ray, how far the final state moves under ulp-level step-factor changes
+    (the oracle with libm pow and with random-ulp pow), precomputed here so
+    the full-size GPU test need not re-trace on the host."""
+    import warnings
+
+    out = {}
+    for j, job in enumerate(workloads.uneven_mix()):
+        _, s0 = _job_state(job)
+        rng = np.random.default_rng(3000 + j)
+        idx = np.sort(rng.choice(s0.shape[0], size=min(per_job, s0.shape[0]), replace=False))
+        state0 = s0[idx].copy()
+        res, kw = _trace_ref(ref_trace, ref_kerr, job, state0)
+        states = np.asarray(res.states)
+        spread = np.zeros(len(idx))
+        st = orc.Kerr(job.mass, job.spin)
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore", RuntimeWarning)
+            for fn in (orc.step_factor_libm, orc.step_factor_ulp_noise(1),
+                       orc.step_factor_ulp_noise(2)):
+                alt = orc.trace_rays(st, state0, factor_fn=fn, **kw)
+                for b in (slice(0, 1), slice(1, 4), slice(4, 5), slice(5, 8)):
+                    d = np.linalg.norm(alt.states[:, b] - states[:, b], axis=-1) / np.maximum(
+                        np.linalg.norm(states[:, b], axis=-1), 1.0)
+                    spread = np.maximum(spread, np.nan_to_num(d, nan=np.inf))
+        out[f"j{j}_index"] = idx
+        out[f"j{j}_state0"] = state0
+        out[f"j{j}_states"] = states
+        out[f"j{j}_status"] = np.asarray(res.status)
+        out[f"j{j}_steps"] = np.asarray(res.steps)
+        out[f"j{j}_spread"] = spread
+        print("uneven job", j, job.rtol, np.bincount(res.status, minlength=7), "max spread",
+              spread.max(), flush=True)
+    np.savez_compressed(os.path.join(HERE, "uneven_all.npz"), **out)
+
+
 def hd_subsample(ref_trace, ref_kerr, per_frame=96):
     """Seeded rays of each 1920x1080 frame (spins x inclinations)."""
     out = {}
@@ -339,8 +377,8 @@ def c5_orbits(integrate_dense, geodesic_rhs, ref_kerr, n_pick=8):
 
 def main():
     ref_root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
-    parts = sys.argv[2:] or ["png", "edge", "edge_r8", "fans", "small", "uneven", "hd", "dense",
-                             "c5"]
+    parts = sys.argv[2:] or ["png", "edge", "edge_r8", "fans", "small", "uneven", "uneven_all",
+                             "hd", "dense", "c5"]
     sys.path.insert(0, ref_root)
     from blackhorizon.geodesics import geodesic_rhs
     from blackhorizon.integrators import integrate_dense
@@ -362,6 +400,8 @@ def main():
         uneven_subsample(trace_rays, KerrSpacetime)
     if "hd" in parts:
         hd_subsample(trace_rays, KerrSpacetime)
+    if "uneven_all" in parts:
+        uneven_all(trace_rays, KerrSpacetime)
     if "dense" in parts:
         dense_orbits(integrate_dense, geodesic_rhs, KerrSpacetime)
     if "c5" in parts:

@@ -86,6 +86,23 @@ def test_uneven_mix_subsample_matches_reference():
                             g[f"j{j}_status"], g[f"j{j}_steps"], label=f"job {j}")
 
 
+def test_uneven_all_jobs_match_reference():
+    """12 rays of each of the 64 uneven-mix jobs (uneven_all.npz): the oracle
+    reproduces the reference's statuses, steps and final states, every ray
+    within 10x its stored host-to-host spread where that exceeds 1e-7, and
+    bit for bit on every ray whose spread is zero."""
+    g = golden("uneven_all.npz")
+    for j, job in enumerate(workloads.uneven_mix()):
+        res = orc.trace_rays(orc.Kerr(spin=job.spin), g[f"j{j}_state0"], job.escape_radius,
+                             max_steps=job.max_steps, rtol=job.rtol, atol=job.atol,
+                             capture_margin=job.capture_margin)
+        assert_trace_parity(res.states, res.status, res.steps, g[f"j{j}_states"],
+                            g[f"j{j}_status"], g[f"j{j}_steps"], label=f"job {j}",
+                            spread=g[f"j{j}_spread"])
+        exact = g[f"j{j}_spread"] == 0.0
+        np.testing.assert_array_equal(res.states[exact], g[f"j{j}_states"][exact])
+
+
 # --- closed forms of the reference's test_shadow.py -------------------------
 
 def fan(st, offsets, distance=1000.0):
