@@ -813,6 +813,42 @@ def run_orbits(args) -> dict:
     return out
 
 
+def run_fit(args) -> dict:
+    """Spin and inclination fit (adjoint.fit_shadow) of a --fit-image^2 camera
+    at r_obs = 40 M, FOV 30 deg, from (a, inc) = (0.7, 45 deg) to a target
+    image traced at (0.75, 50 deg): Adam on the shadow-fit loss with the
+    adjoint's gradients, 60 iterations, step sizes (0.005, 0.5 deg) decayed
+    by 0.93 per iteration."""
+    import math
+
+    import torch
+
+    from blackhorizon_amd import _lib, adjoint
+
+    _lib.require_device()
+    size = args.fit_image
+    rp = 1.0 + math.sqrt(1.0 - 0.75 ** 2)
+    with torch.no_grad():
+        tgt = adjoint.camera_state(0.75, 50.0, 40.0, 30.0, size, size)
+        Yt, st_t = adjoint.trace_terminal(tgt, 0.75, adjoint.TraceSettings(
+            escape_radius=45.0, rtol=1e-8, atol=1e-10, capture_margin=0.01 / rp, K=32))
+        dirs_t = adjoint.terminal_directions(Yt, 0.75)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = adjoint.fit_shadow(dirs_t, st_t, 0.7, 45.0, 40.0, 30.0, size, size)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    first, last = hist[0], hist[-1]
+    return dict(rays=size * size, iterations=len(hist) - 1, seconds=wall,
+                ms_per_iteration=wall * 1e3 / (len(hist) - 1),
+                start=dict(spin=first.spin, inclination=first.inclination, loss=first.loss),
+                final=dict(spin=last.spin, inclination=last.inclination, loss=last.loss,
+                           mismatched=last.mismatched),
+                target=dict(spin=0.75, inclination=50.0),
+                trajectory=[[h.iteration, round(h.spin, 6), round(h.inclination, 4),
+                             round(h.loss, 4)] for h in hist[::5]])
+
+
 def run_dense(args) -> dict:
     """integrate_dense (integrators.py:127-174) on the device: --orbits
     near-circular orbits around a = 0.9 outside r = 15 M (2% velocity
@@ -856,12 +892,13 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="hd",
                     choices=["hd", "hd09", "small", "uneven", "shadowfit", "orbits", "rk4",
-                             "offline", "shader", "dense"])
+                             "offline", "shader", "dense", "fit"])
     ap.add_argument("--dense-lambda", type=float, default=100.0,
                     help="dense: proper time of the batch-shared integration")
     ap.add_argument("--orbits", type=int, default=65536, help="orbits: number of orbits")
     ap.add_argument("--proper-time", type=float, default=1.0e4, help="orbits: proper time")
     ap.add_argument("--fit-size", type=int, default=512, help="shadowfit: image side")
+    ap.add_argument("--fit-image", type=int, default=96, help="fit: image side")
     ap.add_argument("--offline-size", type=int, default=1920,
                     help="offline: frame width (height = 5/8 of it)")
     ap.add_argument("--batch-order", default="longest_first", choices=["longest_first", "given"],
@@ -890,6 +927,11 @@ def main(argv=None) -> int:
                                    "measured; run with --gpus 1"}))
         return 2
 
+    if args.workload == "fit":
+        res = run_fit(args)
+        print(json.dumps({"metric": "spin and inclination fit to a synthetic shadow",
+                          "workload": "fit", **res}), flush=True)
+        return 0
     if args.workload == "dense":
         res = run_dense(args)
         print(json.dumps({"metric": "integrate_dense (batch-shared step) ray-steps/s",

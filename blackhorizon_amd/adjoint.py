@@ -387,3 +387,65 @@ def shadow_loss(terminal: torch.Tensor, status: torch.Tensor, spin, target_dirs:
     if ray_weights is not None:
         per_ray = per_ray * ray_weights
     return per_ray.sum()
+
+
+# ---------------------------------------------------------------------------
+# spin and inclination fit to a shadow image
+# ---------------------------------------------------------------------------
+
+@dataclass
+class FitStep:
+    """One iteration of fit_shadow: parameters and loss before the update,
+    the adjoint gradients, and how many scored rays are misclassified."""
+
+    iteration: int
+    spin: float
+    inclination: float
+    loss: float
+    grad_spin: float
+    grad_inclination: float
+    mismatched: int
+
+
+def fit_shadow(target_dirs: torch.Tensor, target_status: torch.Tensor, spin0: float,
+               inclination0: float, distance: float, fov_degrees: float, width: int,
+               height: int, steps: int = 60, lr: tuple = (0.005, 0.5), gamma: float = 0.93,
+               escape_radius: float = 45.0, rtol: float = 1e-8, atol: float = 1e-10,
+               K: int = 32, mode: str = "auto") -> list[FitStep]:
+    """Fit spin and inclination (degrees) of a pinhole camera at ``distance``
+    to a target image -- the ``terminal_directions`` and statuses of the
+    target's rays, pixel for pixel -- by Adam on ``shadow_loss`` with the
+    adjoint's gradients: per-parameter step sizes ``lr`` (spin,
+    inclination), decayed by ``gamma`` each iteration; captures at
+    r_+ + 0.01 M for the current spin, spin kept inside |a| < 0.999.
+    Returns the history (the last entry is the final iterate's loss)."""
+    dev = target_dirs.device
+    a = torch.tensor(float(spin0), dtype=torch.float64, device=dev, requires_grad=True)
+    inc = torch.tensor(float(inclination0), dtype=torch.float64, device=dev, requires_grad=True)
+    opt = torch.optim.Adam([{"params": [a], "lr": lr[0]}, {"params": [inc], "lr": lr[1]}])
+    sched = torch.optim.lr_scheduler.ExponentialLR(opt, gamma=gamma)
+    history: list[FitStep] = []
+    for it in range(steps + 1):
+        opt.zero_grad()
+        spin_now = float(a.detach())
+        r_plus = 1.0 + math.sqrt(1.0 - spin_now * spin_now)
+        cfg = TraceSettings(escape_radius=escape_radius, rtol=rtol, atol=atol,
+                            capture_margin=0.01 / r_plus, K=K, mode=mode)
+        y0 = camera_state(a, inc, distance, fov_degrees, width, height, device=dev)
+        Y, st = trace_terminal(y0, a, cfg)
+        loss = shadow_loss(Y, st, a, target_dirs, target_status, state0=y0)
+        final = it == steps
+        if not final:
+            loss.backward()
+        mism = int((((st == 1) & (target_status == 2)) | ((st == 2) & (target_status == 1)))
+                   .sum())
+        history.append(FitStep(it, spin_now, float(inc.detach()), float(loss.detach()),
+                               float(a.grad) if not final else float("nan"),
+                               float(inc.grad) if not final else float("nan"), mism))
+        if final:
+            break
+        opt.step()
+        sched.step()
+        with torch.no_grad():
+            a.clamp_(-0.999, 0.999)
+    return history
