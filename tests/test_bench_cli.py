@@ -30,5 +30,5 @@ def test_more_than_one_gpu_is_refused():
 def test_help_lists_the_workloads():
     p = _run(["--help"])
     assert p.returncode == 0
-    for w in ("hd", "uneven", "shadowfit", "orbits", "rk4", "offline"):
+    for w in ("hd", "uneven", "shadowfit", "orbits", "rk4", "offline", "shader", "dense", "fit"):
         assert w in p.stdout
