@@ -217,11 +217,13 @@ def frame_io(job, mode: str, tune, reps: int = 5) -> dict:
     builds them, camera.py:96-113), the whole state built on the device
     (camera_rays_kernel), the (n, 8) initial states copied host -> device
     from pageable and from pinned memory, and the results (states, status,
-    steps) copied device -> host; plus the frame's trace kernel time and the
-    ray-step rate with both copies included."""
+    steps) copied device -> host as the library returns them to NumPy
+    callers (pinned host memory) and into pageable memory; plus the frame's
+    trace kernel time and the ray-step rate with the library's copies
+    included."""
     import torch
 
-    from blackhorizon_amd import kerr, tracer
+    from blackhorizon_amd import _dev, kerr, tracer
     from blackhorizon_amd.camera import PinholeCamera
 
     st = kerr.KerrSpacetime(job.mass, job.spin)
@@ -266,18 +268,31 @@ def frame_io(job, mode: str, tune, reps: int = 5) -> dict:
     ms_kernel = dev_ms(lambda: plan.launch(tune))
     res = plan.result()  # device tensors, like the state it traced
 
-    def d2h():
-        res.states.cpu()
-        res.status.cpu()
-        res.steps.cpu()
+    def d2h():  # the library's path for NumPy callers (pinned host memory)
+        for t in (res.states, res.status, res.steps):
+            _dev.to_numpy(t)
+
+    def d2h_pageable():
+        for t in (res.states, res.status, res.steps):
+            t.cpu()
     ms_d2h = dev_ms(d2h)
+    ms_d2h_pageable = dev_ms(d2h_pageable)
     steps = int(plan.steps.sum().item())
+    # the whole drop-in call of a NumPy caller: upload, initial-radius check,
+    # trace, results back as NumPy
+    order = torch.from_numpy(cam.center_tile_order(8))
+    ms_numpy_call = host_ms(lambda: tracer.trace_rays(
+        st, s_host, job.escape_radius, job.max_steps, job.rtol, job.atol, job.initial_step,
+        job.max_step, job.min_step, job.capture_margin, mode=mode, tune=tune, ray_order=order))
     return dict(spin=job.spin, inclination=job.inclination_degrees, rays=job.n_rays,
                 ms_camera_directions_host=ms_dirs_host, ms_states_on_device=ms_states_dev,
                 ms_h2d_states_pageable=ms_h2d, ms_h2d_states_pinned=ms_h2d_pinned,
-                ms_d2h_results=ms_d2h, ms_trace_kernel=ms_kernel,
+                ms_d2h_results=ms_d2h, ms_d2h_results_pageable=ms_d2h_pageable,
+                ms_trace_kernel=ms_kernel,
                 ray_steps=steps, ray_steps_per_s_kernel=steps / (ms_kernel * 1e-3),
-                ray_steps_per_s_with_copies=steps / ((ms_h2d + ms_kernel + ms_d2h) * 1e-3))
+                ray_steps_per_s_with_copies=steps / ((ms_h2d + ms_kernel + ms_d2h) * 1e-3),
+                ms_trace_rays_numpy=ms_numpy_call,
+                ray_steps_per_s_trace_rays_numpy=steps / (ms_numpy_call * 1e-3))
 
 
 def run_uneven(args) -> dict:

@@ -63,7 +63,10 @@ def xp_of(array: Array):
 
 
 def to_numpy(array: Array) -> numpy.ndarray:
-    """Host copy as a NumPy array (backend.py:55-59)."""
+    """Host copy as a NumPy array (backend.py:55-59); device tensors come
+    back through pinned host memory."""
     if is_tensor(array):
-        return array.detach().cpu().numpy()
+        from . import _dev
+
+        return _dev.to_numpy(array)
     return numpy.asarray(array)

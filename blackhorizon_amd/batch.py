@@ -182,7 +182,7 @@ class BatchPlan:
         lib = _lib.load()
         stream = _lib.stream_handle()
         for g in self.groups:
-            iters = g.iters.cpu().numpy()
+            iters = _dev.to_numpy(g.iters)
             for k, i in enumerate(g.idx):
                 lo, hi = int(g.offsets_host[k]), int(g.offsets_host[k + 1])
                 n = hi - lo
@@ -192,7 +192,7 @@ class BatchPlan:
                                                      g.soa_out.shape[1], _lib.ptr(states), n,
                                                      stream), "soa_to_aos")
                 ref = self.jobs[i].state0
-                conv = (lambda t: t.cpu().numpy()) if (
+                conv = _dev.to_numpy if (
                     as_numpy or (as_numpy is None and not isinstance(ref, torch.Tensor))) \
                     else (lambda t: t)
                 out[i] = TraceResult(states=conv(states), status=conv(g.status[lo:hi]),
