@@ -425,6 +425,38 @@ def test_trace_tensor_io_and_no_mutation(hip):
     np.testing.assert_array_equal(res.status.cpu().numpy(), g["status"][:256])
 
 
+def test_numpy_results_are_independent_copies(hip):
+    """NumPy callers get their results through pinned host memory reused
+    from torch's cache: they equal the tensor path's bit for bit, are
+    ordinary writable NumPy arrays, and stay unchanged when later calls
+    trace other rays (a reused block would overwrite them)."""
+    import torch
+
+    job = workloads.small_frame()
+    g = golden("small_frame_stride8.npz")
+    st = hip.KerrSpacetime(spin=job.spin)
+    kw = dict(rtol=job.rtol, atol=job.atol, capture_margin=job.capture_margin)
+    first = hip.trace_rays(st, g["state0"][:512], job.escape_radius, **kw)
+    keep = [first.states.copy(), first.status.copy(), first.steps.copy()]
+    dev = hip.trace_rays(st, torch.from_numpy(g["state0"][:512].copy()).cuda(),
+                         job.escape_radius, **kw)
+    np.testing.assert_array_equal(first.states, dev.states.cpu().numpy())
+    np.testing.assert_array_equal(first.status, dev.status.cpu().numpy())
+    np.testing.assert_array_equal(first.steps, dev.steps.cpu().numpy())
+    for _ in range(3):  # same sizes: the likeliest to be handed the same blocks
+        later = hip.trace_rays(st, g["state0"][512:1024], job.escape_radius, **kw)
+        del later
+    other = hip.trace_rays(st, g["state0"][512:1024], job.escape_radius, **kw)
+    np.testing.assert_array_equal(first.states, keep[0])
+    np.testing.assert_array_equal(first.status, keep[1])
+    np.testing.assert_array_equal(first.steps, keep[2])
+    assert not np.array_equal(other.states, first.states)
+    for a in (first.states, first.status, first.steps):
+        assert isinstance(a, np.ndarray) and a.flags.writeable
+    first.states[0, 0] = -1.0  # writable without touching the other results
+    assert other.states[0, 0] != -1.0
+
+
 def test_trace_value_errors(hip):
     st = hip.KerrSpacetime(spin=0.0)
     pos = np.array([[50.0, 0.0, 0.0]])
