@@ -81,6 +81,25 @@ def test_presets_match_adaptive_tracer_on_gpu(hip):
         assert np.array_equal(_black(got.status), _black(truth.status)), preset
 
 
+@pytest.mark.parametrize("mode", ["fast", "precise"])
+def test_near_extremal_high_preset_on_gpu(hip, mode):
+    """test_realtime_reference.py:90-108 with both tracers on the device: at
+    a = 0.998 the HIGH preset shades the fan on both sides of the hole
+    exactly like the adaptive tracer at rtol 1e-10."""
+    from blackhorizon_amd import realtime
+
+    st = orc.Kerr(spin=0.998)
+    offsets = np.linspace(0.03, 0.40, 300)
+    s0, _ = off.equatorial_fan(st, np.concatenate([offsets, -offsets]))
+    hst = hip.KerrSpacetime(spin=0.998)
+    truth = hip.trace_rays(hst, s0, escape_radius=60.0, rtol=1e-10)
+    got = realtime.trace_like_shader(hst, s0, _settings("high", spin=0.998), 60.0, mode=mode)
+    assert np.array_equal(_black(got.status), _black(truth.status))
+    # both classes occur on each side (prograde and retrograde edges)
+    for side in (slice(0, 300), slice(300, 600)):
+        assert 0 < int(_black(got.status[side]).sum()) < 300
+
+
 def test_schwarzschild_boundary_and_plungers_on_gpu(hip):
     from blackhorizon_amd import realtime
 

@@ -170,6 +170,20 @@ def test_shader_schwarzschild_boundary_medium():
     assert abs(b - 3.0 * np.sqrt(3.0)) / (3.0 * np.sqrt(3.0)) < 5e-3
 
 
+def test_shader_near_extremal_high_preset():
+    """test_realtime_reference.py:90-108 on the oracles: at a = 0.998 the
+    HIGH preset shades an equatorial fan like the adaptive tracer at rtol
+    1e-10, with both classes on each side of the hole."""
+    st = orc.Kerr(spin=0.998)
+    offsets = np.linspace(0.03, 0.40, 300)
+    s0, _ = off.equatorial_fan(st, np.concatenate([offsets, -offsets]))
+    truth = orc.trace_rays(st, s0, 60.0, rtol=1e-10)
+    ref = off.trace_like_shader(st, s0, 60.0, *off.PRESETS["high"])
+    assert np.array_equal(_black(ref.status), _black(truth.status))
+    for side in (slice(0, 300), slice(300, 600)):
+        assert 0 < int(_black(ref.status[side]).sum()) < 300
+
+
 def test_shader_plungers_never_fail_numerically():
     st = orc.Kerr(spin=0.998)
     s0, _ = off.equatorial_fan(st, np.linspace(0.001, 0.15, 200))
