@@ -830,10 +830,11 @@ def run_orbits(args) -> dict:
 
 def run_fit(args) -> dict:
     """Spin and inclination fit (adjoint.fit_shadow) of a --fit-image^2 camera
-    at r_obs = 40 M, FOV 30 deg, from (a, inc) = (0.7, 45 deg) to a target
-    image traced at (0.75, 50 deg): Adam on the shadow-fit loss with the
-    adjoint's gradients, 60 iterations, step sizes (0.005, 0.5 deg) decayed
-    by 0.93 per iteration."""
+    (default 512^2, the shadow-fit configuration) at r_obs = 40 M, FOV 30
+    deg, from (a, inc) = (0.7, 45 deg) to a target image traced at (0.75, 50
+    deg): Adam on the shadow-fit loss with the adjoint's gradients, rays near
+    the shadow's edge faded by their capture margin (edge scale 0.05), 60
+    iterations, step sizes (0.005, 0.5 deg) decayed by 0.93 per iteration."""
     import math
 
     import torch
@@ -850,7 +851,8 @@ def run_fit(args) -> dict:
         dirs_t = adjoint.terminal_directions(Yt, 0.75)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    hist = adjoint.fit_shadow(dirs_t, st_t, 0.7, 45.0, 40.0, 30.0, size, size)
+    hist = adjoint.fit_shadow(dirs_t, st_t, 0.7, 45.0, 40.0, 30.0, size, size,
+                              edge_scale=0.05)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     first, last = hist[0], hist[-1]
@@ -859,7 +861,7 @@ def run_fit(args) -> dict:
                 start=dict(spin=first.spin, inclination=first.inclination, loss=first.loss),
                 final=dict(spin=last.spin, inclination=last.inclination, loss=last.loss,
                            mismatched=last.mismatched),
-                target=dict(spin=0.75, inclination=50.0),
+                target=dict(spin=0.75, inclination=50.0), edge_scale=0.05,
                 trajectory=[[h.iteration, round(h.spin, 6), round(h.inclination, 4),
                              round(h.loss, 4)] for h in hist[::5]])
 
@@ -913,7 +915,7 @@ def main(argv=None) -> int:
     ap.add_argument("--orbits", type=int, default=65536, help="orbits: number of orbits")
     ap.add_argument("--proper-time", type=float, default=1.0e4, help="orbits: proper time")
     ap.add_argument("--fit-size", type=int, default=512, help="shadowfit: image side")
-    ap.add_argument("--fit-image", type=int, default=96, help="fit: image side")
+    ap.add_argument("--fit-image", type=int, default=512, help="fit: image side")
     ap.add_argument("--offline-size", type=int, default=1920,
                     help="offline: frame width (height = 5/8 of it)")
     ap.add_argument("--batch-order", default="longest_first", choices=["longest_first", "given"],

@@ -349,6 +349,28 @@ def capture_margin(state0: torch.Tensor, spin, mass: float = 1.0, grid: int = 25
     return _radial_potential(r_eval, a, mass, lam, eta)
 
 
+def edge_weights(state0: torch.Tensor, spin, scale: float, status: torch.Tensor | None = None,
+                 target_status: torch.Tensor | None = None, mass: float = 1.0) -> torch.Tensor:
+    """Per-ray weights (n,) for ``shadow_loss`` that fade out rays near the
+    shadow's edge: min(1, |m| / scale)^2 with m the ray's ``capture_margin``,
+    computed without a graph (the weights are constants of the loss).  Near
+    the critical curve a ray winds around the hole before leaving, so its
+    terminal direction turns through large angles for small parameter
+    changes: per-ray gradients reach ~1e5 there and dominate the sum on
+    fine images.  Rays whose class differs from the target's (given both
+    statuses) keep weight 1: their term is the margin itself and must pull
+    them across the edge."""
+    with torch.no_grad():
+        a = spin.detach() if isinstance(spin, torch.Tensor) else spin
+        m = capture_margin(state0.detach(), a, mass)
+        w = torch.clamp(m.abs() / scale, max=1.0) ** 2
+        if status is not None and target_status is not None:
+            mism = (((status == 1) & (target_status == 2))
+                    | ((status == 2) & (target_status == 1)))
+            w = torch.where(mism, torch.ones_like(w), w)
+    return w
+
+
 def shadow_loss(terminal: torch.Tensor, status: torch.Tensor, spin, target_dirs: torch.Tensor,
                 target_status: torch.Tensor, state0: torch.Tensor | None = None,
                 classes: tuple = (1, 2), margin_weight: float = 1.0,
@@ -411,14 +433,18 @@ def fit_shadow(target_dirs: torch.Tensor, target_status: torch.Tensor, spin0: fl
                inclination0: float, distance: float, fov_degrees: float, width: int,
                height: int, steps: int = 60, lr: tuple = (0.005, 0.5), gamma: float = 0.93,
                escape_radius: float = 45.0, rtol: float = 1e-8, atol: float = 1e-10,
-               K: int = 32, mode: str = "auto") -> list[FitStep]:
+               K: int = 32, mode: str = "auto",
+               edge_scale: float | None = 0.05) -> list[FitStep]:
     """Fit spin and inclination (degrees) of a pinhole camera at ``distance``
     to a target image -- the ``terminal_directions`` and statuses of the
     target's rays, pixel for pixel -- by Adam on ``shadow_loss`` with the
     adjoint's gradients: per-parameter step sizes ``lr`` (spin,
     inclination), decayed by ``gamma`` each iteration; captures at
-    r_+ + 0.01 M for the current spin, spin kept inside |a| < 0.999.
-    Returns the history (the last entry is the final iterate's loss)."""
+    r_+ + 0.01 M for the current spin, spin kept inside |a| < 0.999.  With
+    ``edge_scale`` each iteration weights the rays by ``edge_weights`` at
+    the current parameters (None: unweighted), which keeps the photon
+    ring's windings from steering fine images.  Returns the history (the
+    last entry is the final iterate's loss, weighted as it was fitted)."""
     dev = target_dirs.device
     a = torch.tensor(float(spin0), dtype=torch.float64, device=dev, requires_grad=True)
     inc = torch.tensor(float(inclination0), dtype=torch.float64, device=dev, requires_grad=True)
@@ -433,7 +459,9 @@ def fit_shadow(target_dirs: torch.Tensor, target_status: torch.Tensor, spin0: fl
                             capture_margin=0.01 / r_plus, K=K, mode=mode)
         y0 = camera_state(a, inc, distance, fov_degrees, width, height, device=dev)
         Y, st = trace_terminal(y0, a, cfg)
-        loss = shadow_loss(Y, st, a, target_dirs, target_status, state0=y0)
+        w = (None if edge_scale is None
+             else edge_weights(y0, a, edge_scale, st, target_status))
+        loss = shadow_loss(Y, st, a, target_dirs, target_status, state0=y0, ray_weights=w)
         final = it == steps
         if not final:
             loss.backward()

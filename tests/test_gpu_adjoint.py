@@ -336,26 +336,32 @@ def test_backward_lane_order_does_not_change_gradients(hip):
     assert out[0][1] == out[1][1]
 
 
-def test_spin_and_inclination_fit_recovers_the_target(hip):
-    """adjoint.fit_shadow on a 96 x 96 camera, from (a, inc) = (0.7, 45 deg)
-    to a target traced at (0.75, 50 deg): the loss falls below a tenth of
-    its start and both parameters end near the target (measured: 0.741 and
-    50.21 deg after 60 iterations; spin is the weaker-constrained of the
-    two, as a shadow's shape changes little between a = 0.70 and 0.75)."""
+@pytest.mark.parametrize("size,edge_scale,spin_tol", [(96, None, 0.015), (96, 0.05, 0.015),
+                                                      (512, 0.05, 0.01)])
+def test_spin_and_inclination_fit_recovers_the_target(hip, size, edge_scale, spin_tol):
+    """adjoint.fit_shadow from (a, inc) = (0.7, 45 deg) to a target traced at
+    (0.75, 50 deg): the loss falls below a tenth of its start and both
+    parameters end near the target.  On the 96 x 96 camera with the plain
+    loss (measured: 0.741, 50.21 deg after 60 iterations; spin is the
+    weaker-constrained parameter, a shadow's shape changes little between
+    a = 0.70 and 0.75) and with rays near the shadow's edge faded by their
+    capture margin; on the 512 x 512 camera of the shadow-fit configuration
+    only with that fading (measured: 0.752, 50.12 deg), since without it the
+    photon ring's windings steer the fit away (to 0.684, 45.6 deg)."""
     from blackhorizon_amd import adjoint
 
-    size = 96
     rp = 1.0 + math.sqrt(1.0 - 0.75 ** 2)
     with torch.no_grad():
         tgt = adjoint.camera_state(0.75, 50.0, DIST, FOV, size, size)
         Yt, st_t = adjoint.trace_terminal(tgt, 0.75, adjoint.TraceSettings(
             escape_radius=45.0, rtol=1e-8, atol=1e-10, capture_margin=0.01 / rp, K=32))
         dirs_t = adjoint.terminal_directions(Yt, 0.75)
-    hist = adjoint.fit_shadow(dirs_t, st_t, 0.7, 45.0, DIST, FOV, size, size)
+    hist = adjoint.fit_shadow(dirs_t, st_t, 0.7, 45.0, DIST, FOV, size, size,
+                              edge_scale=edge_scale)
     for h in hist[::10] + hist[-1:]:
         print(f"it {h.iteration}: a {h.spin:.5f} inc {h.inclination:.4f} loss {h.loss:.4e} "
               f"mismatched {h.mismatched}")
     first, last = hist[0], hist[-1]
     assert last.loss < 0.1 * first.loss
-    assert abs(last.spin - 0.75) < 0.015
+    assert abs(last.spin - 0.75) < spin_tol
     assert abs(last.inclination - 50.0) < 0.5

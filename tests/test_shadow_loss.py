@@ -135,3 +135,34 @@ def test_shadow_loss_terms():
     assert abs(float(loss) - float(want)) <= 1e-14 * abs(float(want))
     esc_only = adjoint.shadow_loss(term, status, 0.7, tdirs, tstat, state0=s0, classes=(2,))
     assert abs(float(esc_only) - float(psi2[0])) <= 1e-15
+
+
+def test_edge_weights():
+    """min(1, |m| / scale)^2 of the capture margin, without a graph: at
+    a = 0 from the closed form m = 1 - b^2 / 27; rays whose class differs
+    from the target's keep weight 1; the weighted loss is the per-ray terms
+    scaled by them."""
+    s0 = torch.from_numpy(_camera(0.0, width=64, height=1))
+    b2 = (np.cross(s0[:, 1:4].numpy(), s0[:, 5:8].numpy()) ** 2).sum(-1) / s0[:, 4].numpy() ** 2
+    m = 1.0 - b2 / 27.0
+    spin = torch.tensor(0.0, dtype=torch.float64, requires_grad=True)
+    w = adjoint.edge_weights(s0, spin, 0.05)
+    assert not w.requires_grad
+    np.testing.assert_allclose(w.numpy(), np.minimum(1.0, np.abs(m) / 0.05) ** 2, atol=1e-12)
+    assert float(w.min()) < 0.5 and float(w.max()) == 1.0  # the fan crosses the edge
+    status = torch.where(torch.from_numpy(m) > 0, 1, 2).to(torch.int32)
+    flipped = torch.where(status == 1, 2, 1).to(torch.int32)
+    tstat = torch.where(torch.arange(64) % 2 == 0, status, flipped).to(torch.int32)
+    w2 = adjoint.edge_weights(s0, 0.0, 0.05, status, tstat)
+    mism = (status != tstat).numpy()
+    np.testing.assert_array_equal(w2.numpy()[mism], 1.0)
+    np.testing.assert_array_equal(w2.numpy()[~mism], w.numpy()[~mism])
+    term = s0.clone()
+    term[:, 1:4] *= 1.1
+    dirs = adjoint.terminal_directions(term, 0.0)
+    tdirs = torch.roll(dirs, 3, 0)
+    per = [adjoint.shadow_loss(term[i:i + 1], status[i:i + 1], 0.0, tdirs[i:i + 1],
+                               tstat[i:i + 1], state0=s0[i:i + 1]) for i in range(64)]
+    want = sum(float(w2[i]) * float(per[i]) for i in range(64))
+    got = float(adjoint.shadow_loss(term, status, 0.0, tdirs, tstat, state0=s0, ray_weights=w2))
+    assert abs(got - want) <= 1e-12 * abs(want)
