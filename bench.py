@@ -25,8 +25,9 @@ sweep (``targets``; --no-targets skips them): batched_over_per_job, the
 uneven 64-job mix as one persistent batched plan against one plain launch
 per job, both autotuned (--workload uneven); the 65,536-orbit timelike
 workload's forward time, E / L_z / H drift and adjoint/forward (--workload
-orbits); and one a = 0.9 frame's ray generation (host directions, device
-states) and host<->device copies, reported apart from the kernel time.
+orbits); the 512^2 spin and inclination fit (--workload fit); and one a = 0.9
+frame's ray generation (host directions, device states) and host<->device
+copies, reported apart from the kernel time.
 Launch times are reported as means (roofline, matching rocprof's average)
 and medians.
 """
@@ -1076,6 +1077,11 @@ def main(argv=None) -> int:
             "ham_drift_max", "ham_drift_median", "steps_per_orbit", "ms_recording_forward",
             "ms_adjoint", "adjoint_over_forward", "grad_finite")}
         print("[bench] timelike orbits measured", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+        fit = run_fit(argparse.Namespace(**{**vars(args), "fit_image": 512}))
+        targets["fit"] = {k: fit[k] for k in (
+            "rays", "iterations", "ms_per_iteration", "start", "final", "target", "edge_scale")}
+        print("[bench] 512^2 spin/inclination fit done", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
     adj = None
     if not args.no_adjoint and args.workload in ("hd", "hd09"):
