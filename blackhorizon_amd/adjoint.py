@@ -360,6 +360,8 @@ def edge_weights(state0: torch.Tensor, spin, scale: float, status: torch.Tensor 
     fine images.  Rays whose class differs from the target's (given both
     statuses) keep weight 1: their term is the margin itself and must pull
     them across the edge."""
+    if not scale > 0.0:
+        raise ValueError("edge scale must be positive")
     with torch.no_grad():
         a = spin.detach() if isinstance(spin, torch.Tensor) else spin
         m = capture_margin(state0.detach(), a, mass)

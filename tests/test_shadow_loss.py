@@ -166,3 +166,6 @@ def test_edge_weights():
     want = sum(float(w2[i]) * float(per[i]) for i in range(64))
     got = float(adjoint.shadow_loss(term, status, 0.0, tdirs, tstat, state0=s0, ray_weights=w2))
     assert abs(got - want) <= 1e-12 * abs(want)
+    for bad in (0.0, -0.1, float("nan")):
+        with pytest.raises(ValueError):
+            adjoint.edge_weights(s0, 0.0, bad)
