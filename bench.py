@@ -725,7 +725,7 @@ def run_orbits(args) -> dict:
     initial momenta."""
     import torch
 
-    from blackhorizon_amd import _lib, adjoint, batch, geodesics, kerr, workloads
+    from blackhorizon_amd import _lib, adjoint, batch, geodesics, kerr, tracer, workloads
 
     _lib.require_device()
     jobs = workloads.orbit_batches(n_total=args.orbits, lambda_end=args.proper_time)
@@ -739,20 +739,41 @@ def run_orbits(args) -> dict:
         tjobs.append(batch.TraceJob(st, s0, j.escape_radius, j.max_steps, j.rtol, j.atol,
                                     j.initial_step, j.max_step, j.min_step, j.capture_margin,
                                     j.lambda_end))
-    plan = batch.BatchPlan(tjobs, mode=args.mode)
     cur = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    for _ in range(args.warmup):
-        plan.launch()
-    torch.cuda.synchronize()
-    e0.record(cur)
-    for _ in range(args.steps):
-        plan.launch()
-    e1.record(cur)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / args.steps
-    print("[bench] orbits forward done", file=sys.stderr, flush=True)
+
+    def timed(p):
+        for _ in range(args.warmup):
+            p.launch()
+        torch.cuda.synchronize()
+        e0.record(cur)
+        for _ in range(args.steps):
+            p.launch()
+        e1.record(cur)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.steps
+
+    # one lane per orbit, then a DPP quad of lanes per orbit (the precise
+    # build's quad tracer: the same results, each attempt's latency shorter)
+    plan = batch.BatchPlan(tjobs, mode=args.mode)
+    ms_one_lane = timed(plan)
     res = plan.results(as_numpy=False)
+    del plan
+    print("[bench] orbits forward (one lane per orbit) done", file=sys.stderr, flush=True)
+    ms_quad, quad_identical = None, None
+    if all(tracer.resolve_mode(args.mode, j.rtol) == "precise" for j in tjobs):
+        qplan = batch.BatchPlan(tjobs, mode=args.mode, tune=tracer.Tune(lanes_per_ray=4))
+        ms_quad = timed(qplan)
+        qres = qplan.results(as_numpy=False)
+        del qplan
+        quad_identical = all(
+            torch.equal(a.states.view(torch.int64), b.states.view(torch.int64))
+            and torch.equal(a.status, b.status) and torch.equal(a.steps, b.steps)
+            and torch.equal(a.attempts, b.attempts) and a.iterations == b.iterations
+            for a, b in zip(res, qres))
+        del qres
+        print("[bench] orbits forward (quad per orbit) done", file=sys.stderr, flush=True)
+    ms = ms_quad if ms_quad is not None else ms_one_lane
     # the same orbits in the fast build: every attempt's latency is about
     # half the precise build's; its final states are compared with the
     # precise ones below
@@ -791,6 +812,8 @@ def run_orbits(args) -> dict:
     dh = (d1["ham"] - d0["ham"]).abs()[ok]
     out = dict(orbits=int(s0.shape[0]), jobs=len(jobs), proper_time=args.proper_time,
                ray_steps=steps, ms_forward=ms, ray_steps_per_s=steps / (ms * 1e-3),
+               ms_forward_one_lane=ms_one_lane, ms_forward_quad=ms_quad,
+               quad_results_identical=quad_identical,
                ms_forward_fast_build=ms_fast,
                fast_build_statuses_identical=bool(torch.equal(fstatus, status)),
                fast_vs_precise_state_diff_max=float(fdiff.max()),
@@ -1071,6 +1094,7 @@ def main(argv=None) -> int:
                                                "orbits": 65536, "proper_time": 1.0e4}))
         targets["orbits"] = {k: orb[k] for k in (
             "orbits", "jobs", "proper_time", "ray_steps", "ms_forward", "ray_steps_per_s",
+            "ms_forward_one_lane", "ms_forward_quad", "quad_results_identical",
             "ms_forward_fast_build", "fast_build_statuses_identical",
             "fast_vs_precise_state_diff_max",
             "status_counts", "energy_drift_max", "lz_rel_drift_max", "lz_rel_drift_median",

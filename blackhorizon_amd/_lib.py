@@ -28,7 +28,7 @@ MODE_PRECISE = 1
 DENSE_DONE = 0
 DENSE_CHUNK_FULL = 1
 DENSE_UNDERFLOW = 2
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 
 class TraceParams(ctypes.Structure):
@@ -83,7 +83,7 @@ class TuneParams(ctypes.Structure):
         ("block_size", ctypes.c_int32),
         ("rays_per_lane", ctypes.c_int32),
         ("refill_min", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("lanes_per_ray", ctypes.c_int32),
     ]
 
 
@@ -199,7 +199,8 @@ def stream_handle(stream=None) -> int:
     return int(s.cuda_stream)
 
 
-KERNELS = {"plain": 0, "batched": 1, "record": 2, "adjoint": 3, "tile": 4, "shader": 5}
+KERNELS = {"plain": 0, "batched": 1, "record": 2, "adjoint": 3, "tile": 4, "shader": 5,
+           "quad": 6}
 
 
 def kernel_attributes(kernel: str, mode: str = "fast", block: int = 256) -> dict:

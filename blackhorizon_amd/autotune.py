@@ -10,7 +10,11 @@ same per-ray controller); only the speed does.  The shape has three knobs
   its lanes in turn: 1 = one ray per lane, the hardware dispatcher refills;
   >1 = in-wave refill from the wave's range;
 * ``refill_min`` -- idle lanes a wavefront waits for before refilling (plain
-  kernel with rays_per_lane > 1, and the persistent kernel's global queue).
+  kernel with rays_per_lane > 1, and the persistent kernel's global queue);
+* ``lanes_per_ray`` (batched kernel, precise build) -- 1, or 4 for the quad
+  tracer (one DPP quad of lanes per ray), which shortens each ray's
+  attempts at the cost of throughput: it pays when a few long traces set
+  the launch's time.
 
 ``autotune(plan)`` times every candidate shape on a prepared plan
 (``tracer.TracePlan`` or ``batch.BatchPlan``: device-resident inputs, so only
@@ -54,14 +58,20 @@ def plain_candidates() -> list[Tune]:
     return out
 
 
-def batch_candidates() -> list[Tune]:
-    """Shapes for the persistent batched kernel (rays_per_lane unused)."""
+QUAD_REFILL_MINS = (1, 4, 16)
+
+
+def batch_candidates(quad: bool = False) -> list[Tune]:
+    """Shapes for the persistent batched kernel (rays_per_lane unused); with
+    ``quad`` (a plan with a precise group) also the quad tracer's."""
     out = [Tune()]
     for b in BLOCKS:
         for rm in BATCH_REFILL_MINS:
             t = Tune(b, 1, rm)
             if t not in out:
                 out.append(t)
+    if quad:
+        out += [Tune(256, 1, rm, 4) for rm in QUAD_REFILL_MINS]
     return out
 
 
@@ -167,7 +177,10 @@ def autotune(plan, workload_key: str = "default", cache: str | None = DEFAULT_CA
             plan.tune = best
             return TuneResult(key, best, float(hit["best_ms"]), hit.get("table", []), True)
     if candidates is None:
-        candidates = plain_candidates() if _plan_kind(plan) == "plain" else batch_candidates()
+        if _plan_kind(plan) == "plain":
+            candidates = plain_candidates()
+        else:
+            candidates = batch_candidates(quad=any(g.mode == "precise" for g in plan.groups))
     table = time_candidates(plan, candidates, reps, warmup, timer)
     best, best_ms = select(table)
     res = TuneResult(key, best, best_ms, [(asdict(t), ms) for t, ms in table])

@@ -48,19 +48,45 @@ class TraceJob:
     lambda_end: float = math.inf
 
 
+def _initial_radii(state0) -> numpy.ndarray:
+    """|x| of every initial position, on the host."""
+    pos = state0[:, 1:4]
+    if isinstance(pos, torch.Tensor):
+        pos = pos.detach().to("cpu")
+    return numpy.linalg.norm(numpy.asarray(pos, dtype=numpy.float64), axis=1)
+
+
+def expected_steps(job: TraceJob) -> numpy.ndarray | float:
+    """Expected accepted steps of a job's rays, for ranking only.  A ray
+    running to escape covers its distance at the largest step,
+    ``escape_radius / max_step``, plus the steps near the hole, which a
+    fifth-order controller's step scaling makes grow as rtol^-1/5: one value
+    for the job, ``escape_radius / max_step + 6 rtol^-1/5``.  A ray
+    integrated to an affine end takes ``lambda_end / h`` steps, where the
+    controller's step on a bound orbit of radius r is about one dynamical
+    time r^3/2 scaled by rtol^1/5, capped at max_step: one value per ray,
+    from its initial radius."""
+    if math.isfinite(job.lambda_end):
+        r = numpy.maximum(_initial_radii(job.state0), 1e-3)
+        h = numpy.minimum(job.max_step, r ** 1.5 * job.rtol ** 0.2)
+        return job.lambda_end / h
+    return job.escape_radius / job.max_step + 6.0 * job.rtol ** -0.2
+
+
 def longest_first(jobs: list[TraceJob], offsets) -> numpy.ndarray:
-    """Queue order handing out the jobs with the longest expected traces
-    first (the longest-processing-time rule), so that a launch does not end
-    on a long job that started late.  A ray's step count grows with the
-    distance it must cover at the largest step, ``escape_radius /
-    max_step``, and with the tolerance (a fifth-order method's steps scale
-    as rtol^-1/5); jobs are ranked by ``escape_radius / max_step + 6
-    rtol^-1/5``, ties in the given order, and rays keep their order inside a
-    job.  No result depends on the order."""
-    est = [j.escape_radius / j.max_step + 6.0 * j.rtol ** -0.2 for j in jobs]
-    seq = sorted(range(len(jobs)), key=lambda i: -est[i])
-    parts = [numpy.arange(int(offsets[i]), int(offsets[i + 1]), dtype=numpy.int64) for i in seq]
-    return numpy.concatenate(parts) if parts else numpy.zeros(0, numpy.int64)
+    """Queue order handing out the rays with the longest expected traces
+    first (the longest-processing-time rule, ``expected_steps``), so that a
+    launch does not end on a long trace that started late: jobs traced to
+    escape are ranked as a whole, rays integrated to an affine end (timelike
+    orbits) one by one, innermost first; ties keep the given order.  No
+    result depends on the order."""
+    n = int(offsets[-1]) if len(offsets) else 0
+    est = numpy.empty(n, dtype=numpy.float64)
+    for i, j in enumerate(jobs):
+        lo, hi = int(offsets[i]), int(offsets[i + 1])
+        if hi > lo:
+            est[lo:hi] = expected_steps(j)
+    return numpy.argsort(-est, kind="stable").astype(numpy.int64)
 
 
 class _Group:
@@ -121,6 +147,9 @@ class _Group:
 
     def launch(self, tune: _lib.TuneParams, stream: int) -> None:
         lib = _lib.load()
+        if tune.lanes_per_ray > 1 and self.mode != "precise":
+            # the quad tracer exists in the precise build only
+            tune = _lib.TuneParams(tune.block_size, tune.rays_per_lane, tune.refill_min, 1)
         _lib.check(lib.bh_trace_batched_f64(
             _lib.ptr(self.jobs_dev), _lib.ptr(self.offsets_dev), len(self.idx),
             _lib.MODE_FAST if self.mode == "fast" else _lib.MODE_PRECISE, tune,

@@ -152,6 +152,8 @@ int32_t bh_trace_plain_f64(const bh_trace_params *p, const bh_tune_params *tune,
     if (tune->block_size) shape.block = tune->block_size;
     if (tune->rays_per_lane) shape.rays_per_lane = tune->rays_per_lane;
     if (tune->refill_min) shape.refill_min = tune->refill_min;
+    if (tune->lanes_per_ray != 0 && tune->lanes_per_ray != 1)
+      return fail(BH_EINVAL, "the plain tracer takes lanes_per_ray 1");
   }
   if (shape.block <= 0 || shape.block > 256 || shape.block % 64)
     return fail(BH_EINVAL, "block_size must be a multiple of 64 in [64, 256]");
@@ -185,22 +187,29 @@ int32_t bh_trace_batched_f64(const bh_trace_params *jobs, const int64_t *ray_off
   if (n_total > 0 && (!state_in_soa || !state_out_soa || !status || !steps || !attempts))
     return fail(BH_EINVAL, "NULL buffer");
   LaunchShape shape{256, 1, 1};
+  int lanes = 1;
   if (tune) {
     if (tune->block_size) shape.block = tune->block_size;
     if (tune->refill_min) shape.refill_min = tune->refill_min;
+    if (tune->lanes_per_ray) lanes = tune->lanes_per_ray;
   }
   if (shape.block <= 0 || shape.block > 256 || shape.block % 64)
     return fail(BH_EINVAL, "block_size must be a multiple of 64 in [64, 256]");
-  if (shape.refill_min < 1 || shape.refill_min > 64)
-    return fail(BH_EINVAL, "refill_min must be in [1, 64]");
+  if (lanes != 1 && lanes != 4) return fail(BH_EINVAL, "lanes_per_ray must be 1 or 4");
+  if (lanes == 4 && mode != BH_MODE_PRECISE)
+    return fail(BH_EINVAL, "lanes_per_ray 4 needs the precise build");
+  if (shape.refill_min < 1 || shape.refill_min > 64 / lanes)
+    return fail(BH_EINVAL, lanes == 4 ? "refill_min must be in [1, 16] with lanes_per_ray 4"
+                                      : "refill_min must be in [1, 64]");
   TraceIO io{state_in_soa, ld_in, state_out_soa, ld_out, status, steps, attempts,
              (unsigned long long *)iterations, order, n_total};
   hipStream_t s = (hipStream_t)stream;
-  hipError_t e = mode == BH_MODE_PRECISE
-                     ? precise::launch_trace_batched(jobs, ray_offsets, n_jobs, io,
-                                                     (unsigned long long *)workspace, shape, s)
-                     : fast::launch_trace_batched(jobs, ray_offsets, n_jobs, io,
-                                                  (unsigned long long *)workspace, shape, s);
+  unsigned long long *queue = (unsigned long long *)workspace;
+  hipError_t e =
+      lanes == 4 ? precise::launch_trace_quad(jobs, ray_offsets, n_jobs, io, queue, shape, s)
+      : mode == BH_MODE_PRECISE
+          ? precise::launch_trace_batched(jobs, ray_offsets, n_jobs, io, queue, shape, s)
+          : fast::launch_trace_batched(jobs, ray_offsets, n_jobs, io, queue, shape, s);
   return hip_status(e, "bh_trace_batched_f64");
 }
 
@@ -450,7 +459,9 @@ int32_t bh_dp5_step_f64(double mass, double spin, int32_t mode, double rtol, dou
 int32_t bh_kernel_attributes(int32_t kernel, int32_t mode, int32_t block, int32_t *out) {
   if (!out) return fail(BH_EINVAL, "out is NULL");
   if (mode != BH_MODE_FAST && mode != BH_MODE_PRECISE) return fail(BH_EINVAL, "bad mode");
-  if (kernel < BH_KERNEL_PLAIN || kernel > BH_KERNEL_SHADER) return fail(BH_EINVAL, "bad kernel");
+  if (kernel < BH_KERNEL_PLAIN || kernel > BH_KERNEL_QUAD) return fail(BH_EINVAL, "bad kernel");
+  if (kernel == BH_KERNEL_QUAD && mode != BH_MODE_PRECISE)
+    return fail(BH_EINVAL, "the quad tracer exists in the precise build only");
   if (block <= 0 || block > 256 || block % 64) return fail(BH_EINVAL, "bad block");
   if (kernel == BH_KERNEL_ADJOINT && block != 64) return fail(BH_EINVAL, "the adjoint runs 64-thread blocks");
   hipError_t e = mode == BH_MODE_PRECISE ? precise::kernel_attributes(kernel, block, out)

@@ -235,6 +235,13 @@ struct LaunchShape {
 BH_DECLARE_LAUNCHERS(precise)
 BH_DECLARE_LAUNCHERS(fast)
 
+namespace precise {
+// The batched tracer with a DPP quad of lanes per ray (precise build only).
+hipError_t launch_trace_quad(const bh_trace_params *jobs, const int64_t *offsets, int n_jobs,
+                             const TraceIO &io, unsigned long long *queue,
+                             const LaunchShape &shape, hipStream_t s);
+}  // namespace precise
+
 hipError_t launch_aos_to_soa(const double *aos, double *soa, int64_t ld, int64_t n,
                              hipStream_t s);
 hipError_t launch_soa_to_aos(const double *soa, int64_t ld, double *aos, int64_t n,

@@ -45,14 +45,19 @@ class TraceResult:
 
 @dataclass(frozen=True)
 class Tune:
-    """Launch shape of the plain tracer kernel."""
+    """Launch shape of the tracer kernels: threads per block, rays each lane
+    of the plain tracer takes in turn, idle lanes (quads) a wavefront
+    collects before refilling, and lanes per ray of the batched tracer (4:
+    a DPP quad per ray, precise build only).  No result depends on it."""
 
     block_size: int = 256
     rays_per_lane: int = 1
     refill_min: int = 1
+    lanes_per_ray: int = 1
 
     def as_c(self) -> _lib.TuneParams:
-        return _lib.TuneParams(self.block_size, self.rays_per_lane, self.refill_min, 0)
+        return _lib.TuneParams(self.block_size, self.rays_per_lane, self.refill_min,
+                               self.lanes_per_ray)
 
 
 # Below this rtol the fast build's last-ulp RHS rounding, amplified by the

@@ -30,7 +30,8 @@
 extern "C" {
 #endif
 
-#define BH_ABI_VERSION 2  /* 2: integrate_dense work is 24 ld + 8192 doubles; static-scratch adjoint */
+#define BH_ABI_VERSION 3  /* 2: integrate_dense work is 24 ld + 8192 doubles; static-scratch adjoint;
+                             3: bh_tune_params.lanes_per_ray (quad tracer) */
 
 #define BH_OK 0
 #define BH_EINVAL -1   /* invalid argument (maps to ValueError) */
@@ -81,12 +82,16 @@ typedef struct bh_trace_params {
   double lambda_end;
 } bh_trace_params;
 
-/* Launch shape of the plain tracer.  Zero fields pick defaults. */
+/* Launch shape of the tracers.  Zero fields pick defaults. */
 typedef struct bh_tune_params {
   int32_t block_size;      /* threads per block, multiple of 64 (default 256) */
-  int32_t rays_per_lane;   /* rays each lane traces in turn (>= 1; default 1) */
-  int32_t refill_min;      /* idle lanes a wave collects before refilling (1..64) */
-  int32_t reserved;
+  int32_t rays_per_lane;   /* plain tracer: rays each lane traces in turn (>= 1; default 1) */
+  int32_t refill_min;      /* idle lanes (quads with lanes_per_ray 4) a wave collects
+                              before refilling (1..64; 1..16 for quads) */
+  int32_t lanes_per_ray;   /* batched tracer: 1 (default) or 4 -- the four lanes of a
+                              DPP quad trace one ray (precise build only), for launches
+                              whose time is one long trace's latency; the plain tracer
+                              takes 1 only.  Results are the same either way. */
 } bh_tune_params;
 
 int32_t bh_abi_version(void);
@@ -102,6 +107,7 @@ const char *bh_last_error(void);
 #define BH_KERNEL_ADJOINT 3
 #define BH_KERNEL_TILE 4
 #define BH_KERNEL_SHADER 5
+#define BH_KERNEL_QUAD 6    /* precise build only */
 int32_t bh_kernel_attributes(int32_t kernel, int32_t mode, int32_t block, int32_t *out);
 
 /* Number of rays with r0 >= escape_radius, written to *count_out (device
@@ -141,8 +147,11 @@ int32_t bh_trace_plain_f64(const bh_trace_params *p, const bh_tune_params *tune,
  * workspace: device uint64[1] scratch (the ray-queue counter).
  * Outputs per ray exactly as bh_trace_plain_f64 would give for each job
  * alone; only the schedule differs (lanes refill from a global ray queue as
- * rays retire).  n_jobs <= 256.  Replaces a loop of trace_rays calls
- * (tracer.py:50-157), one per job. */
+ * rays retire).  n_jobs <= 256.  tune->lanes_per_ray 4 (precise mode only)
+ * traces each ray on the four lanes of a DPP quad, one wavefront per SIMD:
+ * same outputs, shorter time per attempt for launches that end on a few long
+ * rays.  Replaces a loop of trace_rays calls (tracer.py:50-157), one per
+ * job. */
 int32_t bh_trace_batched_f64(const bh_trace_params *jobs, const int64_t *ray_offsets,
                              int32_t n_jobs, int32_t mode, const bh_tune_params *tune,
                              const double *state_in_soa, int64_t ld_in,

@@ -239,6 +239,29 @@ std::vector<Case> build_cases() {
     bh_tune_params t = tune_of(96, 0, 0);
     return batched(&p, 1, BH_MODE_FAST, &t, 4, 4);
   }, "block_size must be");
+  add("batched: lanes_per_ray 2", [=] {
+    bh_trace_params p = good_params();
+    bh_tune_params t = tune_of(256, 0, 0);
+    t.lanes_per_ray = 2;
+    return batched(&p, 1, BH_MODE_PRECISE, &t, 4, 4);
+  }, "lanes_per_ray must be 1 or 4");
+  add("batched: quad lanes in the fast build", [=] {
+    bh_trace_params p = good_params();
+    bh_tune_params t = tune_of(256, 0, 0);
+    t.lanes_per_ray = 4;
+    return batched(&p, 1, BH_MODE_FAST, &t, 4, 4);
+  }, "needs the precise build");
+  add("batched: quad refill_min 17", [=] {
+    bh_trace_params p = good_params();
+    bh_tune_params t = tune_of(256, 0, 17);
+    t.lanes_per_ray = 4;
+    return batched(&p, 1, BH_MODE_PRECISE, &t, 4, 4);
+  }, "refill_min must be in [1, 16]");
+  add("plain: lanes_per_ray 4", [=] {
+    bh_tune_params t = tune_of(128, 1, 1);
+    t.lanes_per_ray = 4;
+    return plain(&t, d, 4, 4);
+  }, "lanes_per_ray 1");
 
   // --- adjoint record / backward ---
   auto record = [=](int32_t K, int32_t cap, double *ckpt) {
@@ -297,7 +320,10 @@ std::vector<Case> build_cases() {
   add("attributes: out NULL", [=] { return bh_kernel_attributes(0, 0, 64, nullptr); },
       "out is NULL");
   add("attributes: mode 9", [=] { return bh_kernel_attributes(0, 9, 64, i32); }, "bad mode");
-  add("attributes: kernel 6", [=] { return bh_kernel_attributes(6, 0, 64, i32); }, "bad kernel");
+  add("attributes: kernel 7", [=] { return bh_kernel_attributes(7, 0, 64, i32); }, "bad kernel");
+  add("attributes: quad kernel, fast build", [=] {
+    return bh_kernel_attributes(BH_KERNEL_QUAD, BH_MODE_FAST, 64, i32);
+  }, "precise build only");
   add("attributes: kernel -1", [=] { return bh_kernel_attributes(-1, 0, 64, i32); },
       "bad kernel");
   add("attributes: block 0", [=] { return bh_kernel_attributes(0, 0, 0, i32); }, "bad block");

@@ -74,10 +74,30 @@ def test_negative_max_steps_accepted():
 def test_invalid_tune_rejected():
     lib = _lib.load()
     for tune in (_lib.TuneParams(96, 1, 1, 0), _lib.TuneParams(512, 1, 1, 0),
-                 _lib.TuneParams(256, -1, 1, 0), _lib.TuneParams(256, 1, 65, 0)):
+                 _lib.TuneParams(256, -1, 1, 0), _lib.TuneParams(256, 1, 65, 0),
+                 _lib.TuneParams(256, 1, 1, 4), _lib.TuneParams(256, 1, 1, -1)):
         rc = lib.bh_trace_plain_f64(_params(), tune, None, 0, None, 0, None, None, None, None,
                                     None, 0, None)
         assert rc == _lib.BH_EINVAL
+
+
+@pytest.mark.parametrize("mode,tune,msg", [
+    (_lib.MODE_FAST, (256, 1, 1, 4), "needs the precise build"),
+    (_lib.MODE_PRECISE, (256, 1, 1, 2), "lanes_per_ray must be 1 or 4"),
+    (_lib.MODE_PRECISE, (256, 1, 1, -4), "lanes_per_ray must be 1 or 4"),
+    (_lib.MODE_PRECISE, (256, 1, 17, 4), "refill_min must be in [1, 16]"),
+])
+def test_batched_quad_tune_rejected(mode, tune, msg):
+    """The quad tracer (lanes_per_ray 4) exists in the precise build only
+    and takes at most 16 idle quads per refill; other lane counts are
+    refused before any device work."""
+    lib = _lib.load()
+    jobs = (_lib.TraceParams * 1)(_params())
+    dummy = ctypes.c_void_p(8)  # never dereferenced: the call fails its checks first
+    rc = lib.bh_trace_batched_f64(jobs, dummy, 1, mode, _lib.TuneParams(*tune), None, 0, None, 0,
+                                  None, None, None, dummy, None, dummy, 0, None)
+    assert rc == _lib.BH_EINVAL
+    assert msg in lib.bh_last_error().decode()
 
 
 def test_bad_sizes_rejected():

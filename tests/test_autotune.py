@@ -9,18 +9,24 @@ from blackhorizon_amd import autotune
 from blackhorizon_amd.tracer import Tune
 
 
-@pytest.mark.parametrize("cands", [autotune.plain_candidates(), autotune.batch_candidates()])
+@pytest.mark.parametrize("cands", [autotune.plain_candidates(), autotune.batch_candidates(),
+                                   autotune.batch_candidates(quad=True)])
 def test_candidates_within_launch_limits(cands):
     assert cands[0] == Tune()  # the default wins ties
     assert len(set(cands)) == len(cands)
     for t in cands:
         assert t.block_size % 64 == 0 and 64 <= t.block_size <= 256
-        assert 1 <= t.refill_min <= 64
+        assert 1 <= t.refill_min <= 64 // t.lanes_per_ray
         assert t.rays_per_lane >= 1
+        assert t.lanes_per_ray in (1, 4)
 
 
 def test_batch_candidates_have_one_ray_per_lane():
     assert all(t.rays_per_lane == 1 for t in autotune.batch_candidates())
+    assert all(t.lanes_per_ray == 1 for t in autotune.batch_candidates())
+    quads = [t for t in autotune.batch_candidates(quad=True) if t.lanes_per_ray == 4]
+    assert quads and all(t.refill_min <= 16 for t in quads)
+    assert all(t.lanes_per_ray == 1 for t in autotune.plain_candidates())
 
 
 def test_select_fastest_earliest_on_ties():
