@@ -846,6 +846,9 @@ def run_orbits(args) -> dict:
     torch.cuda.synchronize()
     out.update(ms_recording_forward=e[0].elapsed_time(e[1]), ms_adjoint=e[1].elapsed_time(e[2]),
                adjoint_over_forward=e[1].elapsed_time(e[2]) / ms,
+               # the reverse pass runs one lane per orbit: against the
+               # forward of the same lane layout
+               adjoint_over_one_lane_forward=e[1].elapsed_time(e[2]) / ms_one_lane,
                grad_spin=float(a.grad),
                grad_momentum_norm_median=float(sg.grad[:, 5:8].norm(dim=1).median()),
                grad_finite=bool(torch.isfinite(sg.grad).all()))
@@ -1099,7 +1102,8 @@ def main(argv=None) -> int:
             "fast_vs_precise_state_diff_max",
             "status_counts", "energy_drift_max", "lz_rel_drift_max", "lz_rel_drift_median",
             "ham_drift_max", "ham_drift_median", "steps_per_orbit", "ms_recording_forward",
-            "ms_adjoint", "adjoint_over_forward", "grad_finite")}
+            "ms_adjoint", "adjoint_over_forward", "adjoint_over_one_lane_forward",
+            "grad_finite")}
         print("[bench] timelike orbits measured", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
         fit = run_fit(argparse.Namespace(**{**vars(args), "fit_image": 512}))

@@ -32,7 +32,7 @@ import torch
 
 from . import _lib
 from .kerr import KerrSpacetime
-from .tracer import make_params, trace_rays
+from .tracer import Tune, make_params, resolve_mode, trace_rays
 
 TINY = 1e-30
 CKPT_VALUES = 10              # state (8), controller step size, affine parameter
@@ -44,9 +44,10 @@ class TraceSettings:
     """trace_rays keyword arguments (tracer.py:50-61) plus the checkpoint
     interval K, the checkpoint capacity per ray (None: enough for
     max_steps), the scoring sphere of captured rays, r = score_factor r_+,
-    an optional affine-parameter end, and whether the backward hands rays to
-    wavefront lanes in order of decreasing step count (same results; fewer
-    idle lanes)."""
+    an optional affine-parameter end, and whether the recording forward and
+    the backward hand rays to wavefront lanes in order of decreasing
+    (expected) step count (same results; rays of similar length share a
+    wavefront and finish together)."""
 
     escape_radius: float
     max_steps: int = 20000
@@ -63,6 +64,7 @@ class TraceSettings:
     score_factor: float = 3.0
     lambda_end: float = math.inf
     sort_backward: bool = True
+    sort_forward: bool = True
 
 
 class Recorded:
@@ -80,15 +82,21 @@ class Recorded:
                                   cfg.capture_margin, cfg.mode, cfg.lambda_end)
         self.score_radius = cfg.score_factor * st.outer_horizon_radius
         cap = cfg.cap if cfg.cap is not None else cfg.max_steps // cfg.K + 2
+        order = None
         if cfg.cap is None and cap * CKPT_VALUES * 8 * max(n, 1) > CKPT_BUDGET_BYTES:
             # max_steps would reserve more than the budget: size the
-            # checkpoints from the step counts of a plain forward instead
-            pre = trace_rays(st, state0, cfg.escape_radius, max_steps=cfg.max_steps,
-                             rtol=cfg.rtol, atol=cfg.atol, initial_step=cfg.initial_step,
-                             max_step=cfg.max_step, min_step=cfg.min_step,
-                             capture_margin=cfg.capture_margin, mode=cfg.mode,
-                             lambda_end=cfg.lambda_end)
+            # checkpoints from the step counts of a forward pass instead (its
+            # attempt counts also order the recording forward exactly)
+            pre = self._pre_pass(st, state0, cfg)
             cap = int(pre.steps.max().item()) // cfg.K + 2 if n else 1
+            if cfg.sort_forward and n:
+                order = torch.argsort(pre.attempts, descending=True)
+        elif cfg.sort_forward and n and math.isfinite(cfg.lambda_end):
+            from .batch import TraceJob, expected_steps
+
+            est = expected_steps(TraceJob(st, state0, cfg.escape_radius, rtol=cfg.rtol,
+                                          max_step=cfg.max_step, lambda_end=cfg.lambda_end))
+            order = torch.from_numpy(est).to(dev).argsort(descending=True)
         self.cap = max(int(cap), 1)
         ld = max(n, 1)
         f64 = dict(dtype=torch.float64, device=dev)
@@ -117,10 +125,29 @@ class Recorded:
         _lib.check(lib.bh_trace_record_f64(
             self.params, _lib.ptr(self.soa_in), ld, _lib.ptr(self.soa_out), ld,
             _lib.ptr(self.status), _lib.ptr(self.steps), _lib.ptr(self.attempts),
-            _lib.ptr(self.iters), n, cfg.K, self.cap, self.score_radius, _lib.ptr(self.ckpt),
+            _lib.ptr(self.iters), _lib.ptr(order), n, cfg.K, self.cap, self.score_radius,
+            _lib.ptr(self.ckpt),
             _lib.ptr(self.n_ckpt), _lib.ptr(self.term), _lib.ptr(self.prev),
             _lib.ptr(self.evy1), _lib.ptr(self.ev_step), _lib.ptr(self.theta),
             _lib.ptr(self.event), stream), "trace_record")
+
+    @staticmethod
+    def _pre_pass(st, state0, cfg):
+        """The same trace without recording (statuses, steps, attempts): the
+        quad tracer in the precise build, whose launch ends sooner when a
+        few long rays set its time; one lane per ray in the fast build."""
+        if resolve_mode(cfg.mode, cfg.rtol) == "precise":
+            from .batch import TraceJob, trace_batch
+
+            job = TraceJob(st, state0, cfg.escape_radius, cfg.max_steps, cfg.rtol, cfg.atol,
+                           cfg.initial_step, cfg.max_step, cfg.min_step, cfg.capture_margin,
+                           cfg.lambda_end)
+            return trace_batch([job], mode="precise", tune=Tune(lanes_per_ray=4))[0]
+        return trace_rays(st, state0, cfg.escape_radius, max_steps=cfg.max_steps,
+                          rtol=cfg.rtol, atol=cfg.atol, initial_step=cfg.initial_step,
+                          max_step=cfg.max_step, min_step=cfg.min_step,
+                          capture_margin=cfg.capture_margin, mode=cfg.mode,
+                          lambda_end=cfg.lambda_end)
 
     def terminal(self) -> torch.Tensor:
         return self.term[:, : self.n].t().contiguous()

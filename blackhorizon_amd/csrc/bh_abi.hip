@@ -286,8 +286,9 @@ int32_t bh_adjoint_scratch_doubles(int32_t K) {
 
 int32_t bh_trace_record_f64(const bh_trace_params *p, const double *state_in_soa, int64_t ld_in,
                             double *state_out_soa, int64_t ld_out, int32_t *status,
-                            int64_t *steps, int64_t *attempts, uint64_t *iterations, int64_t n,
-                            int32_t K, int32_t cap, double score_radius, double *ckpt,
+                            int64_t *steps, int64_t *attempts, uint64_t *iterations,
+                            const int64_t *order, int64_t n, int32_t K, int32_t cap,
+                            double score_radius, double *ckpt,
                             int32_t *n_ckpt, double *term, double *prev, double *evy1,
                             int64_t *ev_step, double *theta, int32_t *event, void *stream) {
   int32_t rc = check_params(p);
@@ -301,7 +302,7 @@ int32_t bh_trace_record_f64(const bh_trace_params *p, const double *state_in_soa
   hipStream_t s = (hipStream_t)stream;
   JobConst c = make_job_const(*p);
   TraceIO io{state_in_soa, ld_in, state_out_soa, ld_out, status, steps, attempts,
-             (unsigned long long *)iterations, nullptr, n};
+             (unsigned long long *)iterations, order, n};
   RecordIO rec{ckpt, n_ckpt, term, prev, evy1, ev_step, theta, event, score_radius, cap, K};
   hipError_t e = hipSuccess;
   if (iterations) e = launch_zero_u64((unsigned long long *)iterations, s);

@@ -180,11 +180,15 @@ int32_t bh_adjoint_scratch_doubles(int32_t K);
  *  prev [9][n] / evy1 [8][n]: start state + step size / end state of that
  *  event step; ev_step [n]: its accepted-step index; theta [n]: its dense-
  *  output parameter; event [n]: 0 none, 1 escape, 2 scoring sphere,
- *  -1 checkpoint capacity exceeded (no gradient). */
+ *  -1 checkpoint capacity exceeded (no gradient).
+ * order: optional device int64[n] permutation, lane slot k traces ray
+ * order[k] (NULL = identity); rays of similar length in one wavefront
+ * finish together.  Results land at each ray's own index. */
 int32_t bh_trace_record_f64(const bh_trace_params *p, const double *state_in_soa, int64_t ld_in,
                             double *state_out_soa, int64_t ld_out, int32_t *status,
-                            int64_t *steps, int64_t *attempts, uint64_t *iterations, int64_t n,
-                            int32_t K, int32_t cap, double score_radius, double *ckpt,
+                            int64_t *steps, int64_t *attempts, uint64_t *iterations,
+                            const int64_t *order, int64_t n, int32_t K, int32_t cap,
+                            double score_radius, double *ckpt,
                             int32_t *n_ckpt, double *term, double *prev, double *evy1,
                             int64_t *ev_step, double *theta, int32_t *event, void *stream);
 

@@ -336,6 +336,37 @@ def test_backward_lane_order_does_not_change_gradients(hip):
     assert out[0][1] == out[1][1]
 
 
+@pytest.mark.parametrize("presize", [False, True])
+def test_recording_lane_order_does_not_change_results(hip, presize):
+    """Timelike orbits to a proper-time end, recorded with rays handed to
+    lanes innermost first (the default: by expected steps, or by the
+    checkpoint-sizing pre-pass's exact attempt counts) or in index order:
+    identical terminal points, statuses, step counts and gradients."""
+    from blackhorizon_amd import adjoint, geodesics, kerr, workloads
+
+    pos, vel = workloads.near_circular_orbits(300, seed=11)
+    st = kerr.KerrSpacetime(1.0, 0.9)
+    p = torch.from_numpy(pos).cuda()
+    s0 = geodesics.build_state(p, geodesics.timelike_momentum(st, p, torch.from_numpy(vel).cuda()))
+    gY = torch.from_numpy(np.random.default_rng(5).normal(size=(300, 8))).cuda()
+    # max_steps 1e8 makes the default checkpoint capacity exceed the budget,
+    # so the capacity (and the order) come from the pre-pass
+    extra = dict(max_steps=100_000_000) if presize else dict(max_steps=200_000)
+    out = []
+    for sort in (True, False):
+        cfg = adjoint.TraceSettings(escape_radius=1000.0, rtol=1e-10, atol=1e-12,
+                                    initial_step=0.05, min_step=1e-14, K=16, lambda_end=300.0,
+                                    sort_forward=sort, **extra)
+        sg = s0.clone().requires_grad_(True)
+        ag = torch.tensor(0.9, dtype=torch.float64, device="cuda", requires_grad=True)
+        Y, status = adjoint.trace_terminal(sg, ag, cfg)
+        (Y * gY).sum().backward()
+        out.append((Y.detach().cpu().numpy(), status.cpu().numpy(), sg.grad.cpu().numpy(),
+                    float(ag.grad)))
+    for a, b in zip(out[0], out[1]):
+        np.testing.assert_array_equal(np.asarray(a), np.asarray(b))
+
+
 @pytest.mark.parametrize("size,edge_scale,spin_tol", [(96, None, 0.015), (96, 0.05, 0.015),
                                                       (512, 0.05, 0.01)])
 def test_spin_and_inclination_fit_recovers_the_target(hip, size, edge_scale, spin_tol):
