@@ -85,6 +85,44 @@ def test_rhs(hip, spin):
     np.testing.assert_allclose(fast, ref, rtol=1e-11, atol=1e-13)
 
 
+def _bits_equal(a, b, label):
+    a, b = np.asarray(a), np.asarray(b)
+    same = a.view(np.int64) == b.view(np.int64)
+    assert np.all(same | (np.isnan(a) & np.isnan(b))), label
+
+
+@pytest.mark.parametrize("spin", [0.0, 0.9, -0.998])
+def test_precise_signed_zeros_bitwise(hip, spin):
+    """States with exact +-0.0 coordinates and momenta (the origin, the
+    axis, the equatorial plane, zero momentum components): the precise
+    build's right-hand side, geometry and diagnostics equal the oracle's
+    raw bit patterns, signed zeros included -- NumPy's three-term sums start
+    from +0.0, so an all -0.0 sum is +0.0 there and on the device."""
+    from blackhorizon_amd import geodesics
+
+    rng = np.random.default_rng(17)
+    s = rng.normal(size=(4096, 8)) * rng.choice([1e-3, 1.0, 10.0, 300.0], size=(4096, 1))
+    s[::3, 3] = 0.0
+    s[::5, 1] = -0.0
+    s[::7, 2] = 0.0
+    s[::11, 1:4] = 0.0
+    s[::13, 5] = -0.0
+    s[::17, 6] = -0.0
+    s[::19, 7] = 0.0
+    s[::23, 5:8] = -0.0
+    ost, st = orc.Kerr(spin=spin), hip.KerrSpacetime(spin=spin)
+    with np.errstate(all="ignore"):
+        ref = orc.geodesic_rhs(ost, s)
+        _bits_equal(geodesics.geodesic_rhs(st, s, mode="precise"), ref, "rhs")
+        got = st.geometry(s[:, 1], s[:, 2], s[:, 3], gradients=True, mode="precise")
+        want = ost.geometry(s[:, 1], s[:, 2], s[:, 3], gradients=True)
+        for f in ("radius", "h", "l", "grad_h", "grad_l"):
+            _bits_equal(getattr(got, f), getattr(want, f), f)
+        _bits_equal(geodesics.hamiltonian(st, s), orc.hamiltonian(ost, s), "hamiltonian")
+        _bits_equal(geodesics.coordinate_velocity(st, s), orc.coordinate_velocity(ost, s),
+                    "coordinate_velocity")
+
+
 @pytest.mark.parametrize("spin", [0.9, -0.5])
 def test_dp5_step(hip, spin):
     from blackhorizon_amd import integrators
