@@ -122,6 +122,7 @@ class Recorded:
         self.ev_step = torch.empty(n, dtype=torch.int64, device=dev)
         self.theta = torch.empty(n, **f64)
         self.event = torch.empty(n, dtype=torch.int32, device=dev)
+        self.workspace = torch.zeros(1, dtype=torch.int64, device=dev)
         _lib.check(lib.bh_trace_record_f64(
             self.params, _lib.ptr(self.soa_in), ld, _lib.ptr(self.soa_out), ld,
             _lib.ptr(self.status), _lib.ptr(self.steps), _lib.ptr(self.attempts),
@@ -129,7 +130,7 @@ class Recorded:
             _lib.ptr(self.ckpt),
             _lib.ptr(self.n_ckpt), _lib.ptr(self.term), _lib.ptr(self.prev),
             _lib.ptr(self.evy1), _lib.ptr(self.ev_step), _lib.ptr(self.theta),
-            _lib.ptr(self.event), stream), "trace_record")
+            _lib.ptr(self.event), _lib.ptr(self.workspace), stream), "trace_record")
 
     @staticmethod
     def _pre_pass(st, state0, cfg):

@@ -82,6 +82,21 @@ constexpr int64_t MAX_RAYS = (int64_t)1 << 36;
 
 static bool rays_ok(int64_t n) { return n >= 0 && n <= MAX_RAYS; }
 
+// Compute units of the current device (cached per thread and device).
+static hipError_t device_cus(int *cus) {
+  static thread_local int cached_device = -1, cached_cus = 0;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev != cached_device) {
+    e = hipDeviceGetAttribute(&cached_cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    cached_device = dev;
+  }
+  *cus = cached_cus;
+  return hipSuccess;
+}
+
 static int32_t check_mode(int32_t mode) {
   if (mode != BH_MODE_FAST && mode != BH_MODE_PRECISE)
     return fail(BH_EINVAL, "mode must be BH_MODE_FAST or BH_MODE_PRECISE");
@@ -290,14 +305,15 @@ int32_t bh_trace_record_f64(const bh_trace_params *p, const double *state_in_soa
                             const int64_t *order, int64_t n, int32_t K, int32_t cap,
                             double score_radius, double *ckpt,
                             int32_t *n_ckpt, double *term, double *prev, double *evy1,
-                            int64_t *ev_step, double *theta, int32_t *event, void *stream) {
+                            int64_t *ev_step, double *theta, int32_t *event, uint64_t *workspace,
+                            void *stream) {
   int32_t rc = check_params(p);
   if (rc) return rc;
   if (!rays_ok(n) || ld_in < n || ld_out < n) return fail(BH_EINVAL, "bad n / ld");
   if (K < 1 || K > MAX_ADJOINT_K || cap < 1)
     return fail(BH_EINVAL, "need 1 <= K <= 2^20 and cap >= 1");
   if (n > 0 && (!state_in_soa || !state_out_soa || !status || !steps || !attempts || !ckpt ||
-                !n_ckpt || !term || !prev || !evy1 || !ev_step || !theta || !event))
+                !n_ckpt || !term || !prev || !evy1 || !ev_step || !theta || !event || !workspace))
     return fail(BH_EINVAL, "NULL buffer");
   hipStream_t s = (hipStream_t)stream;
   JobConst c = make_job_const(*p);
@@ -306,9 +322,17 @@ int32_t bh_trace_record_f64(const bh_trace_params *p, const double *state_in_soa
   RecordIO rec{ckpt, n_ckpt, term, prev, evy1, ev_step, theta, event, score_radius, cap, K};
   hipError_t e = hipSuccess;
   if (iterations) e = launch_zero_u64((unsigned long long *)iterations, s);
-  if (e == hipSuccess)
-    e = p->mode == BH_MODE_PRECISE ? precise::launch_trace_record(c, io, rec, s)
-                                   : fast::launch_trace_record(c, io, rec, s);
+  int cus = 0;
+  if (e == hipSuccess && n > 0 && p->mode == BH_MODE_PRECISE) e = device_cus(&cus);
+  if (e == hipSuccess && n > 0) {
+    // precise recordings of at most one wavefront per SIMD are as long as
+    // their longest rays: a quad per ray shortens each attempt (same bits)
+    if (p->mode == BH_MODE_PRECISE && (n + 63) / 64 <= (int64_t)cus * 4)
+      e = precise::launch_trace_record_quad(c, io, rec, (unsigned long long *)workspace, cus, s);
+    else
+      e = p->mode == BH_MODE_PRECISE ? precise::launch_trace_record(c, io, rec, s)
+                                     : fast::launch_trace_record(c, io, rec, s);
+  }
   return hip_status(e, "bh_trace_record_f64");
 }
 

@@ -240,6 +240,10 @@ namespace precise {
 hipError_t launch_trace_quad(const bh_trace_params *jobs, const int64_t *offsets, int n_jobs,
                              const TraceIO &io, unsigned long long *queue,
                              const LaunchShape &shape, hipStream_t s);
+// The recording forward with a quad per ray, one wavefront per SIMD on
+// `cus` compute units.
+hipError_t launch_trace_record_quad(const JobConst &c, const TraceIO &io, const RecordIO &rec,
+                                    unsigned long long *queue, int cus, hipStream_t s);
 }  // namespace precise
 
 hipError_t launch_aos_to_soa(const double *aos, double *soa, int64_t ld, int64_t n,

@@ -183,14 +183,19 @@ int32_t bh_adjoint_scratch_doubles(int32_t K);
  *  -1 checkpoint capacity exceeded (no gradient).
  * order: optional device int64[n] permutation, lane slot k traces ray
  * order[k] (NULL = identity); rays of similar length in one wavefront
- * finish together.  Results land at each ray's own index. */
+ * finish together.  Results land at each ray's own index.
+ * workspace: device uint64[1] scratch (a ray-queue counter): a precise-mode
+ * recording of at most one wavefront per SIMD runs on DPP quads of lanes,
+ * one ray per quad, handed out in `order` -- the same outputs, bit for bit,
+ * sooner when a few long rays set the time. */
 int32_t bh_trace_record_f64(const bh_trace_params *p, const double *state_in_soa, int64_t ld_in,
                             double *state_out_soa, int64_t ld_out, int32_t *status,
                             int64_t *steps, int64_t *attempts, uint64_t *iterations,
                             const int64_t *order, int64_t n, int32_t K, int32_t cap,
                             double score_radius, double *ckpt,
                             int32_t *n_ckpt, double *term, double *prev, double *evy1,
-                            int64_t *ev_step, double *theta, int32_t *event, void *stream);
+                            int64_t *ev_step, double *theta, int32_t *event, uint64_t *workspace,
+                            void *stream);
 
 /* Reverse pass: given gterm [8][n] = dL/d(terminal point), writes
  * gy0 [8][n] = dL/d(initial state) and gspin [n] = per-ray dL/d(spin),

@@ -267,7 +267,7 @@ std::vector<Case> build_cases() {
   auto record = [=](int32_t K, int32_t cap, double *ckpt) {
     bh_trace_params p = good_params();
     return bh_trace_record_f64(&p, d, 4, d, 4, i32, i64, i64, u64, nullptr, 4, K, cap, 20.0, ckpt,
-                               i32, d, d, d, i64, d, i32, nullptr);
+                               i32, d, d, d, i64, d, i32, u64, nullptr);
   };
   add("record: K 0", [=] { return record(0, 4, d); }, "1 <= K <= 2^20");
   add("record: K 2^20+1", [=] { return record((1 << 20) + 1, 4, d); }, "1 <= K <= 2^20");

@@ -365,6 +365,44 @@ def test_backward_stage_layouts_give_the_same_bits(hip, mode):
     assert torch.isfinite(gy_sub).all()
 
 
+def test_recording_on_quads_gives_the_same_bits(hip):
+    """A precise recording of at most one wavefront per SIMD runs on DPP
+    quads (one ray per quad), a larger one on one lane per ray.  Every
+    101st ray of a 70,000-ray camera recorded alone (quads) equals the
+    whole camera's recording (one lane) bit for bit: final states, status,
+    steps, attempts, checkpoints, event steps, terminal points, theta and
+    event kinds."""
+    from blackhorizon_amd import adjoint
+
+    props = torch.cuda.get_device_properties(0)
+    big = 280 * 250
+    assert -(-big // 64) > props.multi_processor_count * 4  # one lane per ray
+    s0 = adjoint.camera_state(SPIN, INC, DIST, FOV, 280, 250).detach()
+    cfg = _settings("precise", cap=400)
+    sub = torch.arange(0, big, 101, device="cuda")
+    a = adjoint.Recorded(s0, SPIN, cfg)
+    b = adjoint.Recorded(s0[sub].contiguous(), SPIN, cfg)
+
+    def bits(t):
+        t = t.detach().contiguous().cpu().numpy()
+        return t.view(np.int64) if t.dtype == np.float64 else t
+
+    n = sub.shape[0]
+    for name in ("soa_out", "term", "prev", "evy1"):
+        np.testing.assert_array_equal(bits(getattr(a, name)[:, sub]),
+                                      bits(getattr(b, name)[:, :n]), err_msg=name)
+    for name in ("status", "steps", "attempts", "n_ckpt", "ev_step", "theta", "event"):
+        np.testing.assert_array_equal(bits(getattr(a, name)[sub]), bits(getattr(b, name)),
+                                      err_msg=name)
+    nck = b.n_ckpt.cpu().numpy()
+    assert nck.max() <= 400 and (nck >= 1).all()
+    ca, cb = a.ckpt[:, :, sub].cpu().numpy(), b.ckpt[:, :, :n].cpu().numpy()
+    for k in range(n):  # the checkpoints each ray wrote
+        np.testing.assert_array_equal(ca[: nck[k], :, k].view(np.int64),
+                                      cb[: nck[k], :, k].view(np.int64), err_msg=f"ray {k}")
+    assert int(a.iters.item()) >= int(b.iters.item())
+
+
 @pytest.mark.parametrize("presize", [False, True])
 def test_recording_lane_order_does_not_change_results(hip, presize):
     """Timelike orbits to a proper-time end, recorded with rays handed to
