@@ -37,6 +37,8 @@ from .tracer import Tune, make_params, resolve_mode, trace_rays
 TINY = 1e-30
 CKPT_VALUES = 10              # state (8), controller step size, affine parameter
 CKPT_BUDGET_BYTES = 16 << 30  # checkpoint memory reserved from max_steps alone
+TRAJ_VALUES = 9               # per stored step: start state (8) and step size
+TRAJ_BUDGET_BYTES = 64 << 30  # largest trajectory store chosen automatically
 
 
 @dataclass
@@ -47,7 +49,15 @@ class TraceSettings:
     an optional affine-parameter end, and whether the recording forward and
     the backward hand rays to wavefront lanes in order of decreasing
     (expected) step count (same results; rays of similar length share a
-    wavefront and finish together)."""
+    wavefront and finish together).
+
+    ``trajectory``: store every accepted step (72 bytes each) instead of
+    checkpoints, so the backward reverses the stored steps without replaying
+    them -- same gradients, bit for bit.  None chooses it when a pre-pass
+    sizes the recording anyway (checkpoint capacity from max_steps over its
+    budget) and the store fits TRAJ_BUDGET_BYTES and half the free device
+    memory; True always (with a pre-pass for the exact per-ray sizes); False
+    never."""
 
     escape_radius: float
     max_steps: int = 20000
@@ -65,6 +75,7 @@ class TraceSettings:
     lambda_end: float = math.inf
     sort_backward: bool = True
     sort_forward: bool = True
+    trajectory: bool | None = None
 
 
 class Recorded:
@@ -83,14 +94,25 @@ class Recorded:
         self.score_radius = cfg.score_factor * st.outer_horizon_radius
         cap = cfg.cap if cfg.cap is not None else cfg.max_steps // cfg.K + 2
         order = None
-        if cfg.cap is None and cap * CKPT_VALUES * 8 * max(n, 1) > CKPT_BUDGET_BYTES:
-            # max_steps would reserve more than the budget: size the
-            # checkpoints from the step counts of a forward pass instead (its
-            # attempt counts also order the recording forward exactly)
+        self.traj = self.traj_off = None
+        if cfg.trajectory or (cfg.cap is None and
+                              cap * CKPT_VALUES * 8 * max(n, 1) > CKPT_BUDGET_BYTES):
+            # max_steps would reserve more than the budget (or a trajectory
+            # store is asked for): size the recording from the step counts
+            # of a forward pass instead (its attempt counts also order the
+            # recording forward exactly)
             pre = self._pre_pass(st, state0, cfg)
             cap = int(pre.steps.max().item()) // cfg.K + 2 if n else 1
             if cfg.sort_forward and n:
                 order = torch.argsort(pre.attempts, descending=True)
+            total = int(pre.steps.sum().item()) if n else 0
+            store = total * TRAJ_VALUES * 8
+            if cfg.trajectory or (cfg.trajectory is None and store <= TRAJ_BUDGET_BYTES and
+                                  store <= torch.cuda.mem_get_info(dev)[0] // 2):
+                self.traj_off = torch.zeros(n + 1, dtype=torch.int64, device=dev)
+                torch.cumsum(pre.steps.to(dev, torch.int64), 0, out=self.traj_off[1:])
+                self.traj = torch.empty(max(total, 1) * TRAJ_VALUES, dtype=torch.float64,
+                                        device=dev)
         elif cfg.sort_forward and n and math.isfinite(cfg.lambda_end):
             from .batch import TraceJob, expected_steps
 
@@ -114,8 +136,6 @@ class Recorded:
         self.steps = torch.empty(n, dtype=torch.int64, device=dev)
         self.attempts = torch.empty(n, dtype=torch.int64, device=dev)
         self.iters = torch.zeros(1, dtype=torch.int64, device=dev)
-        self.ckpt = torch.empty((self.cap, CKPT_VALUES, ld), **f64)
-        self.n_ckpt = torch.empty(n, dtype=torch.int32, device=dev)
         self.term = torch.empty((8, ld), **f64)
         self.prev = torch.empty((9, ld), **f64)
         self.evy1 = torch.empty((8, ld), **f64)
@@ -123,6 +143,19 @@ class Recorded:
         self.theta = torch.empty(n, **f64)
         self.event = torch.empty(n, dtype=torch.int32, device=dev)
         self.workspace = torch.zeros(1, dtype=torch.int64, device=dev)
+        if self.traj is not None:
+            self.ckpt = self.n_ckpt = None
+            _lib.check(lib.bh_trace_record_trajectory_f64(
+                self.params, _lib.ptr(self.soa_in), ld, _lib.ptr(self.soa_out), ld,
+                _lib.ptr(self.status), _lib.ptr(self.steps), _lib.ptr(self.attempts),
+                _lib.ptr(self.iters), _lib.ptr(order), n, _lib.ptr(self.traj_off),
+                _lib.ptr(self.traj), self.score_radius, _lib.ptr(self.term),
+                _lib.ptr(self.prev), _lib.ptr(self.evy1), _lib.ptr(self.ev_step),
+                _lib.ptr(self.theta), _lib.ptr(self.event), _lib.ptr(self.workspace), stream),
+                "trace_record_trajectory")
+            return
+        self.ckpt = torch.empty((self.cap, CKPT_VALUES, ld), **f64)
+        self.n_ckpt = torch.empty(n, dtype=torch.int32, device=dev)
         _lib.check(lib.bh_trace_record_f64(
             self.params, _lib.ptr(self.soa_in), ld, _lib.ptr(self.soa_out), ld,
             _lib.ptr(self.status), _lib.ptr(self.steps), _lib.ptr(self.attempts),
@@ -163,20 +196,27 @@ class Recorded:
         ld = max(n, 1)
         dev = self.term.device
         if int((self.event < 0).sum().item()) > 0:
-            raise RuntimeError("a ray exceeded the checkpoint capacity; raise cap")
+            raise RuntimeError("a ray exceeded its checkpoint capacity or trajectory slots; raise cap")
         gterm = torch.zeros((8, ld), dtype=torch.float64, device=dev)
         gterm[:, :n] = g_terminal.t()
         gy0 = torch.empty((8, ld), dtype=torch.float64, device=dev)
         gspin = torch.empty(ld, dtype=torch.float64, device=dev)
+        # longest reversals first, rays of similar length in one wavefront
+        order = torch.argsort(self.ev_step, descending=True) if self.cfg.sort_backward and n \
+            else None
+        if self.traj is not None:
+            _lib.check(lib.bh_trace_adjoint_trajectory_f64(
+                self.params, n, self.score_radius, _lib.ptr(self.traj_off), _lib.ptr(self.traj),
+                _lib.ptr(self.event), _lib.ptr(self.ev_step), _lib.ptr(self.evy1),
+                _lib.ptr(self.prev), _lib.ptr(self.theta), _lib.ptr(gterm), _lib.ptr(gy0),
+                _lib.ptr(gspin), _lib.ptr(order), _lib.stream_handle()), "trace_adjoint_trajectory")
+            return gy0[:, :n].t().contiguous(), gspin[:n]
         per_ray = lib.bh_adjoint_scratch_doubles(self.cfg.K)
         if per_ray < 0:
             raise ValueError("checkpoint interval K out of range")
         # one block of per_ray x 64 doubles per wavefront of 64 rays
         scratch = torch.empty(per_ray * (-(-n // 64) * 64 or 64), dtype=torch.float64,
                               device=dev)
-        # longest reversals first, rays of similar length in one wavefront
-        order = torch.argsort(self.ev_step, descending=True) if self.cfg.sort_backward and n \
-            else None
         _lib.check(lib.bh_trace_adjoint_f64(
             self.params, n, self.cfg.K, self.score_radius, _lib.ptr(self.ckpt),
             _lib.ptr(self.event), _lib.ptr(self.ev_step), _lib.ptr(self.evy1),

@@ -252,10 +252,11 @@ static int32_t launch_adjoint(const bh_trace_params *p, int64_t n, int32_t K, do
                               const double *ckpt, const int32_t *event, const int64_t *ev_step,
                               const double *evy1, const double *prev, const double *theta,
                               const double *gterm, double *gy0, double *gspin, double *scratch,
-                              const int64_t *order, void *stream, const char *name) {
+                              const int64_t *order, const double *traj, const int64_t *traj_off,
+                              void *stream, const char *name) {
   JobConst c = make_job_const(*p);
-  AdjointIO a{ckpt,  event, ev_step, evy1,  prev,         theta, gterm, gy0,
-              gspin, scratch, order, score_radius, n, K};
+  AdjointIO a{ckpt,  event,   ev_step, evy1, prev,     theta,        gterm, gy0,
+              gspin, scratch, order,   traj, traj_off, score_radius, n,     K};
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = p->mode == BH_MODE_PRECISE ? precise::launch_trace_adjoint(c, a, s)
                                             : fast::launch_trace_adjoint(c, a, s);
@@ -299,6 +300,29 @@ int32_t bh_adjoint_scratch_doubles(int32_t K) {
   return K < 1 || K > MAX_ADJOINT_K ? -1 : (int32_t)adjoint_scratch_per_ray(K);
 }
 
+// The recording launch shared by the checkpoint and trajectory entries.
+static int32_t launch_record(const bh_trace_params *p, const TraceIO &io, const RecordIO &rec,
+                             uint64_t *workspace, void *stream, const char *name) {
+  const int64_t n = io.n;
+  unsigned long long *iterations = io.iterations;
+  hipStream_t s = (hipStream_t)stream;
+  JobConst c = make_job_const(*p);
+  hipError_t e = hipSuccess;
+  if (iterations) e = launch_zero_u64((unsigned long long *)iterations, s);
+  int cus = 0;
+  if (e == hipSuccess && n > 0 && p->mode == BH_MODE_PRECISE) e = device_cus(&cus);
+  if (e == hipSuccess && n > 0) {
+    // precise recordings of at most one wavefront per SIMD are as long as
+    // their longest rays: a quad per ray shortens each attempt (same bits)
+    if (p->mode == BH_MODE_PRECISE && (n + 63) / 64 <= (int64_t)cus * 4)
+      e = precise::launch_trace_record_quad(c, io, rec, (unsigned long long *)workspace, cus, s);
+    else
+      e = p->mode == BH_MODE_PRECISE ? precise::launch_trace_record(c, io, rec, s)
+                                     : fast::launch_trace_record(c, io, rec, s);
+  }
+  return hip_status(e, name);
+}
+
 int32_t bh_trace_record_f64(const bh_trace_params *p, const double *state_in_soa, int64_t ld_in,
                             double *state_out_soa, int64_t ld_out, int32_t *status,
                             int64_t *steps, int64_t *attempts, uint64_t *iterations,
@@ -315,26 +339,32 @@ int32_t bh_trace_record_f64(const bh_trace_params *p, const double *state_in_soa
   if (n > 0 && (!state_in_soa || !state_out_soa || !status || !steps || !attempts || !ckpt ||
                 !n_ckpt || !term || !prev || !evy1 || !ev_step || !theta || !event || !workspace))
     return fail(BH_EINVAL, "NULL buffer");
-  hipStream_t s = (hipStream_t)stream;
-  JobConst c = make_job_const(*p);
   TraceIO io{state_in_soa, ld_in, state_out_soa, ld_out, status, steps, attempts,
              (unsigned long long *)iterations, order, n};
-  RecordIO rec{ckpt, n_ckpt, term, prev, evy1, ev_step, theta, event, score_radius, cap, K};
-  hipError_t e = hipSuccess;
-  if (iterations) e = launch_zero_u64((unsigned long long *)iterations, s);
-  int cus = 0;
-  if (e == hipSuccess && n > 0 && p->mode == BH_MODE_PRECISE) e = device_cus(&cus);
-  if (e == hipSuccess && n > 0) {
-    // precise recordings of at most one wavefront per SIMD are as long as
-    // their longest rays: a quad per ray shortens each attempt (same bits)
-    if (p->mode == BH_MODE_PRECISE && (n + 63) / 64 <= (int64_t)cus * 4)
-      e = precise::launch_trace_record_quad(c, io, rec, (unsigned long long *)workspace, cus, s);
-    else
-      e = p->mode == BH_MODE_PRECISE ? precise::launch_trace_record(c, io, rec, s)
-                                     : fast::launch_trace_record(c, io, rec, s);
-  }
-  return hip_status(e, "bh_trace_record_f64");
+  RecordIO rec{ckpt,    n_ckpt,  term,         prev, evy1, ev_step, theta,
+               event,   nullptr, nullptr,      score_radius, cap,     K};
+  return launch_record(p, io, rec, workspace, stream, "bh_trace_record_f64");
 }
+
+int32_t bh_trace_record_trajectory_f64(
+    const bh_trace_params *p, const double *state_in_soa, int64_t ld_in, double *state_out_soa,
+    int64_t ld_out, int32_t *status, int64_t *steps, int64_t *attempts, uint64_t *iterations,
+    const int64_t *order, int64_t n, const int64_t *traj_off, double *traj, double score_radius,
+    double *term, double *prev, double *evy1, int64_t *ev_step, double *theta, int32_t *event,
+    uint64_t *workspace, void *stream) {
+  int32_t rc = check_params(p);
+  if (rc) return rc;
+  if (!rays_ok(n) || ld_in < n || ld_out < n) return fail(BH_EINVAL, "bad n / ld");
+  if (n > 0 && (!state_in_soa || !state_out_soa || !status || !steps || !attempts || !traj_off ||
+                !traj || !term || !prev || !evy1 || !ev_step || !theta || !event || !workspace))
+    return fail(BH_EINVAL, "NULL buffer");
+  TraceIO io{state_in_soa, ld_in, state_out_soa, ld_out, status, steps, attempts,
+             (unsigned long long *)iterations, order, n};
+  RecordIO rec{nullptr, nullptr, term,     prev,         evy1, ev_step, theta,
+               event,   traj,    traj_off, score_radius, 0,    1};
+  return launch_record(p, io, rec, workspace, stream, "bh_trace_record_trajectory_f64");
+}
+
 
 int32_t bh_trace_adjoint_f64(const bh_trace_params *p, int64_t n, int32_t K, double score_radius,
                              const double *ckpt, const int32_t *event, const int64_t *ev_step,
@@ -346,7 +376,8 @@ int32_t bh_trace_adjoint_f64(const bh_trace_params *p, int64_t n, int32_t K, dou
   if (rc) return rc;
   if (n > 0 && !scratch) return fail(BH_EINVAL, "NULL buffer");
   return launch_adjoint(p, n, K, score_radius, ckpt, event, ev_step, evy1, prev, theta, gterm,
-                        gy0, gspin, scratch, order, stream, "bh_trace_adjoint_f64");
+                        gy0, gspin, scratch, order, nullptr, nullptr, stream,
+                        "bh_trace_adjoint_f64");
 }
 
 int32_t bh_trace_adjoint_static_f64(const bh_trace_params *p, int64_t n, int32_t K,
@@ -370,7 +401,26 @@ int32_t bh_trace_adjoint_static_f64(const bh_trace_params *p, int64_t n, int32_t
   rc = static_scratch(&ptr);
   if (rc) return rc;
   return launch_adjoint(p, n, K, score_radius, ckpt, event, ev_step, evy1, prev, theta, gterm,
-                        gy0, gspin, ptr, order, stream, "bh_trace_adjoint_static_f64");
+                        gy0, gspin, ptr, order, nullptr, nullptr, stream,
+                        "bh_trace_adjoint_static_f64");
+}
+
+int32_t bh_trace_adjoint_trajectory_f64(const bh_trace_params *p, int64_t n,
+                                        double score_radius, const int64_t *traj_off,
+                                        const double *traj, const int32_t *event,
+                                        const int64_t *ev_step, const double *evy1,
+                                        const double *prev, const double *theta,
+                                        const double *gterm, double *gy0, double *gspin,
+                                        const int64_t *order, void *stream) {
+  int32_t rc = check_params(p);
+  if (rc) return rc;
+  if (!rays_ok(n)) return fail(BH_EINVAL, "bad n");
+  if (n > 0 && (!traj_off || !traj || !event || !ev_step || !evy1 || !prev || !theta || !gterm ||
+                !gy0 || !gspin))
+    return fail(BH_EINVAL, "NULL buffer");
+  return launch_adjoint(p, n, 1, score_radius, nullptr, event, ev_step, evy1, prev, theta, gterm,
+                        gy0, gspin, nullptr, order, traj, traj_off, stream,
+                        "bh_trace_adjoint_trajectory_f64");
 }
 
 int32_t bh_rhs_vjp_f64(double mass, double spin, const double *state, const double *v,

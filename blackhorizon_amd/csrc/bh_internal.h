@@ -73,10 +73,19 @@ struct RecordIO {
   int64_t *ev_step;  // [n]: accepted-step index at the end of the event step
   double *theta;     // [n]: dense-output parameter of the event in that step
   int32_t *event;    // [n]: 0 none, 1 escape sphere, 2 scoring sphere, -1 checkpoint overflow
+  // optional trajectory (instead of checkpoints): ray i's accepted step s
+  // (start state and step size, TRAJ_VALUES doubles) at
+  // traj[(traj_off[i] + s) * TRAJ_VALUES], s < traj_off[i + 1] - traj_off[i]
+  double *traj;
+  const int64_t *traj_off;
   double score_radius;
   int32_t cap;
   int32_t K;
 };
+
+// Values per stored trajectory step: the step's start state (8) and its
+// accepted step size.
+constexpr int TRAJ_VALUES = 9;
 
 struct AdjointIO {
   const double *ckpt;
@@ -90,6 +99,8 @@ struct AdjointIO {
   double *gspin;        // [n] per-ray spin adjoint
   double *scratch;      // [ceil(n / 64)][9 K][64]
   const int64_t *order; // optional [n]: lane slot -> ray
+  const double *traj;   // optional trajectory (RecordIO::traj): no replay
+  const int64_t *traj_off;
   double score_radius;
   int64_t n;
   int32_t K;

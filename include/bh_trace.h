@@ -197,6 +197,22 @@ int32_t bh_trace_record_f64(const bh_trace_params *p, const double *state_in_soa
                             int64_t *ev_step, double *theta, int32_t *event, uint64_t *workspace,
                             void *stream);
 
+/* bh_trace_record_f64 storing every accepted step instead of checkpoints,
+ * for bh_trace_adjoint_trajectory_f64 (no replay in the reverse pass).
+ * traj_off: device int64[n + 1], non-decreasing; ray i's steps go to
+ * traj[(traj_off[i] + s) * 9 + 0..8] (start state of accepted step s, then
+ * its step size), s < traj_off[i + 1] - traj_off[i]; traj holds
+ * traj_off[n] * 9 device doubles.  A ray with more accepted steps than its
+ * slots gets event -1.  Sizing the slots from a plain trace's per-ray step
+ * counts is exact (the recording takes the same steps).  Every other output
+ * is bh_trace_record_f64's, bit for bit. */
+int32_t bh_trace_record_trajectory_f64(
+    const bh_trace_params *p, const double *state_in_soa, int64_t ld_in, double *state_out_soa,
+    int64_t ld_out, int32_t *status, int64_t *steps, int64_t *attempts, uint64_t *iterations,
+    const int64_t *order, int64_t n, const int64_t *traj_off, double *traj, double score_radius,
+    double *term, double *prev, double *evy1, int64_t *ev_step, double *theta, int32_t *event,
+    uint64_t *workspace, void *stream);
+
 /* Reverse pass: given gterm [8][n] = dL/d(terminal point), writes
  * gy0 [8][n] = dL/d(initial state) and gspin [n] = per-ray dL/d(spin),
  * differentiating every accepted step with its step size held fixed (each
@@ -235,6 +251,20 @@ int32_t bh_trace_adjoint_static_f64(const bh_trace_params *p, int64_t n, int32_t
                                     const double *evy1, const double *prev,
                                     const double *theta, const double *gterm, double *gy0,
                                     double *gspin, const int64_t *order, void *stream);
+
+/* bh_trace_adjoint_f64 over a trajectory stored by
+ * bh_trace_record_trajectory_f64 (same traj_off and traj): each step's
+ * start state and size are read instead of replayed, so there is no
+ * scratch and no checkpoint interval.  Results equal bh_trace_adjoint_f64's
+ * on a checkpoint recording of the same rays, bit for bit.  Rays with event
+ * -1, or fewer stored steps than ev_step, get NaN. */
+int32_t bh_trace_adjoint_trajectory_f64(const bh_trace_params *p, int64_t n,
+                                        double score_radius, const int64_t *traj_off,
+                                        const double *traj, const int32_t *event,
+                                        const int64_t *ev_step, const double *evy1,
+                                        const double *prev, const double *theta,
+                                        const double *gterm, double *gy0, double *gspin,
+                                        const int64_t *order, void *stream);
 
 /* Vector-Jacobian product of geodesic_rhs (geodesics.py:42-74) at n states
  * (rows): zbar = J^T v (n, 8) and abar = v . d f / d spin (n). */

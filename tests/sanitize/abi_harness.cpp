@@ -282,6 +282,33 @@ std::vector<Case> build_cases() {
   add("adjoint: n INT64_MAX", [=] { return adjoint(BIG, 8, d); }, "bad n");
   add("adjoint: K INT32_MAX", [=] { return adjoint(4, INT32_MAX, d); }, "1 <= K <= 2^20");
   add("adjoint: scratch NULL", [=] { return adjoint(4, 8, nullptr); }, "NULL buffer");
+  auto record_traj = [=](int64_t n, const int64_t *off, double *traj) {
+    bh_trace_params p = good_params();
+    return bh_trace_record_trajectory_f64(&p, d, 4, d, 4, i32, i64, i64, u64, nullptr, n, off,
+                                          traj, 20.0, d, d, d, i64, d, i32, u64, nullptr);
+  };
+  add("record trajectory: n -1", [=] { return record_traj(-1, i64, d); }, "bad n / ld");
+  add("record trajectory: n > ld", [=] { return record_traj(5, i64, d); }, "bad n / ld");
+  add("record trajectory: offsets NULL", [=] { return record_traj(4, nullptr, d); },
+      "NULL buffer");
+  add("record trajectory: store NULL", [=] { return record_traj(4, i64, nullptr); },
+      "NULL buffer");
+  auto adjoint_traj = [=](int64_t n, const int64_t *off, const double *traj) {
+    bh_trace_params p = good_params();
+    return bh_trace_adjoint_trajectory_f64(&p, n, 20.0, off, traj, i32, i64, d, d, d, d, d, d,
+                                           nullptr, nullptr);
+  };
+  add("adjoint trajectory: n INT64_MAX", [=] { return adjoint_traj(BIG, i64, d); }, "bad n");
+  add("adjoint trajectory: offsets NULL", [=] { return adjoint_traj(4, nullptr, d); },
+      "NULL buffer");
+  add("adjoint trajectory: store NULL", [=] { return adjoint_traj(4, i64, nullptr); },
+      "NULL buffer");
+  add("adjoint trajectory: mode 3", [=] {
+    bh_trace_params p = good_params();
+    p.mode = 3;
+    return bh_trace_adjoint_trajectory_f64(&p, 4, 20.0, i64, d, i32, i64, d, d, d, d, d, d,
+                                           nullptr, nullptr);
+  }, "mode must be");
   add("rhs_vjp: spin NaN", [=] {
     return bh_rhs_vjp_f64(1.0, __builtin_nan(""), d, d, 4, d, d, nullptr);
   }, "Kerr bound");

@@ -434,6 +434,119 @@ def test_recording_lane_order_does_not_change_results(hip, presize):
         np.testing.assert_array_equal(np.asarray(a), np.asarray(b))
 
 
+def _bits(t):
+    t = t.detach().contiguous().cpu().numpy()
+    return t.view(np.int64) if t.dtype == np.float64 else t
+
+
+@pytest.mark.parametrize("mode,width,height", [("precise", 64, 48), ("precise", 280, 250),
+                                               ("fast", 64, 48)])
+def test_trajectory_store_gives_the_same_bits(hip, mode, width, height):
+    """A recording that stores every accepted step (trajectory store, no
+    checkpoints) and the backward that reverses the stored steps without a
+    replay give what the checkpoint recording and its replaying backward
+    give, bit for bit: final states, statuses, step and attempt counts,
+    event steps, terminal points, theta, event kinds, and the gradients of
+    the initial states and the spin.  Camera rays escape and are captured
+    (both event kinds); 3,072 precise rays record on quads, 70,000 on one
+    lane per ray.  Every checkpoint equals the stored start state of its
+    step, and the stored step sizes sum to the affine length of each ray's
+    steps."""
+    from blackhorizon_amd import adjoint
+
+    s0 = adjoint.camera_state(SPIN, INC, DIST, FOV, width, height).detach()
+    n = s0.shape[0]
+    a = adjoint.Recorded(s0, SPIN, _settings(mode, cap=400, trajectory=False))
+    b = adjoint.Recorded(s0, SPIN, _settings(mode, cap=400, trajectory=True))
+    assert a.traj is None and b.traj is not None and b.ckpt is None
+    for name in ("soa_out", "term", "prev", "evy1"):
+        np.testing.assert_array_equal(_bits(getattr(a, name)[:, :n]),
+                                      _bits(getattr(b, name)[:, :n]), err_msg=name)
+    for name in ("status", "steps", "attempts", "ev_step", "theta", "event"):
+        np.testing.assert_array_equal(_bits(getattr(a, name)), _bits(getattr(b, name)),
+                                      err_msg=name)
+    ev = b.event.cpu().numpy()
+    assert (ev == 1).any() and (ev == 2).any() and (ev >= 0).all()
+    off = b.traj_off.cpu().numpy()
+    steps = b.steps.cpu().numpy()
+    np.testing.assert_array_equal(np.diff(off), steps)
+    traj = b.traj.cpu().numpy()[: off[-1] * adjoint.TRAJ_VALUES].reshape(-1, 9)
+    ck, nck = a.ckpt.cpu().numpy(), a.n_ckpt.cpu().numpy()
+    K = a.cfg.K
+    for k in range(0, n, 37):
+        for j in range(1, nck[k]):  # checkpoint j: the state after j K steps
+            np.testing.assert_array_equal(ck[j, :8, k].view(np.int64),
+                                          traj[off[k] + j * K, :8].view(np.int64))
+        np.testing.assert_array_equal(traj[off[k], :8].view(np.int64),
+                                      _bits(s0[k]).view(np.int64))
+    gY = torch.from_numpy(np.random.default_rng(3).normal(size=(n, 8))).cuda()
+    ga, sa = a.backward(gY)
+    gb, sb = b.backward(gY)
+    assert torch.isfinite(ga).all()
+    np.testing.assert_array_equal(_bits(ga), _bits(gb))
+    np.testing.assert_array_equal(_bits(sa), _bits(sb))
+
+
+def test_trajectory_slots_too_few(hip):
+    """A ray given fewer trajectory slots than it takes steps is recorded
+    as event -1 (nothing written past its slots); the backward gives NaN to
+    a ray whose slots are fewer than its event step, the other rays their
+    gradients."""
+    from blackhorizon_amd import _lib, adjoint
+
+    s0 = adjoint.camera_state(SPIN, INC, DIST, FOV, 16, 12).detach()
+    n = s0.shape[0]
+    b = adjoint.Recorded(s0, SPIN, _settings("precise", cap=400, trajectory=True))
+    lib = _lib.require_device()
+    steps = b.steps.clone()
+    assert int(steps[5]) > 2
+    steps[5] -= 1
+    off = torch.zeros(n + 1, dtype=torch.int64, device="cuda")
+    torch.cumsum(steps, 0, out=off[1:])
+    traj = torch.full_like(b.traj, 7.0)
+    ws = torch.zeros(1, dtype=torch.int64, device="cuda")
+    out = {k: torch.empty_like(getattr(b, k)) for k in ("soa_out", "status", "steps", "attempts",
+                                                        "iters", "term", "prev", "evy1",
+                                                        "ev_step", "theta", "event")}
+    ld = n
+    _lib.check(lib.bh_trace_record_trajectory_f64(
+        b.params, _lib.ptr(b.soa_in), ld, _lib.ptr(out["soa_out"]), ld, _lib.ptr(out["status"]),
+        _lib.ptr(out["steps"]), _lib.ptr(out["attempts"]), _lib.ptr(out["iters"]), None, n,
+        _lib.ptr(off), _lib.ptr(traj), b.score_radius, _lib.ptr(out["term"]),
+        _lib.ptr(out["prev"]), _lib.ptr(out["evy1"]), _lib.ptr(out["ev_step"]),
+        _lib.ptr(out["theta"]), _lib.ptr(out["event"]), _lib.ptr(ws), _lib.stream_handle()),
+        "record")
+    ev = out["event"].cpu().numpy()
+    want = b.event.cpu().numpy().copy()
+    want[5] = -1
+    np.testing.assert_array_equal(ev, want)
+    # nothing lands past ray 5's slots: ray 6's steps are the full recording's,
+    # and the store's last step (no ray's slot now) is untouched
+    o, ob = off.cpu().numpy(), b.traj_off.cpu().numpy()
+    np.testing.assert_array_equal(_bits(traj[o[6] * 9: o[7] * 9]),
+                                  _bits(b.traj[ob[6] * 9: ob[7] * 9]))
+    assert float(traj[o[-1] * 9:].min()) == 7.0 == float(traj[o[-1] * 9:].max())
+    # the backward on the full recording with ray 5's slots cut below its event step
+    ev_step = b.ev_step.cpu().numpy()
+    assert ev_step[5] >= 2
+    off2 = b.traj_off.clone()
+    off2[5] = off2[6] - int(ev_step[5]) + 1  # ray 5 keeps ev_step - 1 slots, ray 4 gains
+    gterm = torch.zeros((8, n), dtype=torch.float64, device="cuda")
+    gterm[1] = 1.0
+    gy0 = torch.empty((8, n), dtype=torch.float64, device="cuda")
+    gspin = torch.empty(n, dtype=torch.float64, device="cuda")
+    _lib.check(lib.bh_trace_adjoint_trajectory_f64(
+        b.params, n, b.score_radius, _lib.ptr(off2), _lib.ptr(b.traj), _lib.ptr(b.event),
+        _lib.ptr(b.ev_step), _lib.ptr(b.evy1), _lib.ptr(b.prev), _lib.ptr(b.theta),
+        _lib.ptr(gterm), _lib.ptr(gy0), _lib.ptr(gspin), None, _lib.stream_handle()), "adjoint")
+    g = gy0.cpu().numpy()
+    ref, _ = b.backward(gterm[:, :n].t().contiguous())
+    ref = ref.cpu().numpy()
+    assert np.isnan(g[:, 5]).all() and np.isnan(float(gspin[5]))
+    keep = [k for k in range(n) if k != 5]
+    np.testing.assert_array_equal(g[:, keep].T.view(np.int64), ref[keep].view(np.int64))
+
+
 @pytest.mark.parametrize("size,edge_scale,spin_tol", [(96, None, 0.015), (96, 0.05, 0.015),
                                                       (512, 0.05, 0.01)])
 def test_spin_and_inclination_fit_recovers_the_target(hip, size, edge_scale, spin_tol):
