@@ -105,10 +105,10 @@ SIGNATURES = {
     "bh_trace_record_f64": [ctypes.POINTER(TraceParams), _P, _I64, _P, _I64, _P, _P, _P, _P, _P,
                             _I64, _I32, _I32, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "bh_trace_record_trajectory_f64": [ctypes.POINTER(TraceParams), _P, _I64, _P, _I64, _P, _P,
-                                       _P, _P, _P, _I64, _P, _P, _D, _P, _P, _P, _P, _P, _P, _P,
-                                       _P],
-    "bh_trace_adjoint_trajectory_f64": [ctypes.POINTER(TraceParams), _I64, _D, _P, _P, _P, _P,
-                                        _P, _P, _P, _P, _P, _P, _P, _P],
+                                       _P, _P, _P, _I64, _P, _I64, _P, _P, _D, _P, _P, _P, _P,
+                                       _P, _P, _P, _P],
+    "bh_trace_adjoint_trajectory_f64": [ctypes.POINTER(TraceParams), _I64, _D, _P, _I64, _P, _P,
+                                        _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "bh_trace_adjoint_f64": [ctypes.POINTER(TraceParams), _I64, _I32, _D, _P, _P, _P, _P, _P, _P,
                              _P, _P, _P, _P, _P, _P],
     "bh_trace_adjoint_static_f64": [ctypes.POINTER(TraceParams), _I64, _I32, _D, _P, _P, _P, _P,

@@ -38,6 +38,7 @@ TINY = 1e-30
 CKPT_VALUES = 10              # state (8), controller step size, affine parameter
 CKPT_BUDGET_BYTES = 16 << 30  # checkpoint memory reserved from max_steps alone
 TRAJ_VALUES = 9               # per stored step: start state (8) and step size
+TRAJ_PAGE_STEPS = 32          # accepted steps per page of the store (BH_TRAJ_PAGE_STEPS)
 TRAJ_BUDGET_BYTES = 64 << 30  # largest trajectory store chosen automatically
 
 
@@ -51,13 +52,15 @@ class TraceSettings:
     (expected) step count (same results; rays of similar length share a
     wavefront and finish together).
 
-    ``trajectory``: store every accepted step (72 bytes each) instead of
+    ``trajectory``: store every accepted step (72 bytes each, in pages of
+    TRAJ_PAGE_STEPS steps that rays take from a pool as they go) instead of
     checkpoints, so the backward reverses the stored steps without replaying
-    them -- same gradients, bit for bit.  None chooses it when a pre-pass
-    sizes the recording anyway (checkpoint capacity from max_steps over its
-    budget) and the store fits TRAJ_BUDGET_BYTES and half the free device
-    memory; True always (with a pre-pass for the exact per-ray sizes); False
-    never."""
+    them -- same gradients, bit for bit.  The pool is sized from the rays'
+    expected step counts; should the recording find it too small, the
+    exact per-ray counts of a plain trace size it and the recording runs
+    again.  None chooses the store when the estimated pool fits
+    TRAJ_BUDGET_BYTES and half the free device memory; True always; False
+    never (checkpoints)."""
 
     escape_radius: float
     max_steps: int = 20000
@@ -94,31 +97,32 @@ class Recorded:
         self.score_radius = cfg.score_factor * st.outer_horizon_radius
         cap = cfg.cap if cfg.cap is not None else cfg.max_steps // cfg.K + 2
         order = None
-        self.traj = self.traj_off = None
-        if cfg.trajectory or (cfg.cap is None and
-                              cap * CKPT_VALUES * 8 * max(n, 1) > CKPT_BUDGET_BYTES):
-            # max_steps would reserve more than the budget (or a trajectory
-            # store is asked for): size the recording from the step counts
-            # of a forward pass instead (its attempt counts also order the
-            # recording forward exactly)
+        pre = None
+        self.pool = self.page_link = self.last_page = None
+        self.pool_pages = 0
+        pool_pages = None
+        est = self._expected_steps(st, state0, cfg) if n else None
+        if cfg.trajectory is not False:
+            pages = self._pages(est, n)
+            if cfg.trajectory or self._store_fits(pages, dev):
+                pool_pages = pages
+        if pool_pages is not None and not self._store_fits(pool_pages, dev):
+            # a store is asked for beyond the estimate's budget: exact sizes
+            pre = self._pre_pass(st, state0, cfg)
+            pool_pages = self._pages(pre.steps, n)
+        elif pool_pages is None and cfg.cap is None and \
+                cap * CKPT_VALUES * 8 * max(n, 1) > CKPT_BUDGET_BYTES:
+            # max_steps would reserve more checkpoints than the budget: size
+            # them from the step counts of a forward pass instead
             pre = self._pre_pass(st, state0, cfg)
             cap = int(pre.steps.max().item()) // cfg.K + 2 if n else 1
-            if cfg.sort_forward and n:
+        if cfg.sort_forward and n:
+            # longest rays first: the pre-pass's exact attempt counts, else,
+            # for rays traced to an affine end, their expected steps
+            if pre is not None:
                 order = torch.argsort(pre.attempts, descending=True)
-            total = int(pre.steps.sum().item()) if n else 0
-            store = total * TRAJ_VALUES * 8
-            if cfg.trajectory or (cfg.trajectory is None and store <= TRAJ_BUDGET_BYTES and
-                                  store <= torch.cuda.mem_get_info(dev)[0] // 2):
-                self.traj_off = torch.zeros(n + 1, dtype=torch.int64, device=dev)
-                torch.cumsum(pre.steps.to(dev, torch.int64), 0, out=self.traj_off[1:])
-                self.traj = torch.empty(max(total, 1) * TRAJ_VALUES, dtype=torch.float64,
-                                        device=dev)
-        elif cfg.sort_forward and n and math.isfinite(cfg.lambda_end):
-            from .batch import TraceJob, expected_steps
-
-            est = expected_steps(TraceJob(st, state0, cfg.escape_radius, rtol=cfg.rtol,
-                                          max_step=cfg.max_step, lambda_end=cfg.lambda_end))
-            order = torch.from_numpy(est).to(dev).argsort(descending=True)
+            elif math.isfinite(cfg.lambda_end):
+                order = torch.from_numpy(est).to(dev).argsort(descending=True)
         self.cap = max(int(cap), 1)
         ld = max(n, 1)
         f64 = dict(dtype=torch.float64, device=dev)
@@ -142,17 +146,16 @@ class Recorded:
         self.ev_step = torch.empty(n, dtype=torch.int64, device=dev)
         self.theta = torch.empty(n, **f64)
         self.event = torch.empty(n, dtype=torch.int32, device=dev)
-        self.workspace = torch.zeros(1, dtype=torch.int64, device=dev)
-        if self.traj is not None:
+        self.workspace = torch.zeros(2, dtype=torch.int64, device=dev)
+        if pool_pages is not None:
             self.ckpt = self.n_ckpt = None
-            _lib.check(lib.bh_trace_record_trajectory_f64(
-                self.params, _lib.ptr(self.soa_in), ld, _lib.ptr(self.soa_out), ld,
-                _lib.ptr(self.status), _lib.ptr(self.steps), _lib.ptr(self.attempts),
-                _lib.ptr(self.iters), _lib.ptr(order), n, _lib.ptr(self.traj_off),
-                _lib.ptr(self.traj), self.score_radius, _lib.ptr(self.term),
-                _lib.ptr(self.prev), _lib.ptr(self.evy1), _lib.ptr(self.ev_step),
-                _lib.ptr(self.theta), _lib.ptr(self.event), _lib.ptr(self.workspace), stream),
-                "trace_record_trajectory")
+            self._record_store(lib, pool_pages, order, stream)
+            if int(self.workspace[1].item()) > pool_pages:
+                # the estimate fell short: exact sizes from a plain trace
+                pre = self._pre_pass(st, state0, cfg)
+                if cfg.sort_forward:
+                    order = torch.argsort(pre.attempts, descending=True)
+                self._record_store(lib, self._pages(pre.steps, n), order, stream)
             return
         self.ckpt = torch.empty((self.cap, CKPT_VALUES, ld), **f64)
         self.n_ckpt = torch.empty(n, dtype=torch.int32, device=dev)
@@ -164,6 +167,75 @@ class Recorded:
             _lib.ptr(self.n_ckpt), _lib.ptr(self.term), _lib.ptr(self.prev),
             _lib.ptr(self.evy1), _lib.ptr(self.ev_step), _lib.ptr(self.theta),
             _lib.ptr(self.event), _lib.ptr(self.workspace), stream), "trace_record")
+
+    def _record_store(self, lib, pool_pages, order, stream):
+        """The recording forward into a trajectory store of pool_pages pages."""
+        n = self.n
+        ld = max(n, 1)
+        dev = self.term.device
+        self.pool_pages = int(pool_pages)
+        self.pool = self.page_link = None  # release a smaller store first
+        self.pool = torch.empty(max(self.pool_pages, 1) * TRAJ_PAGE_STEPS * TRAJ_VALUES,
+                                dtype=torch.float64, device=dev)
+        self.page_link = torch.empty(max(self.pool_pages, 1), dtype=torch.int64, device=dev)
+        self.last_page = torch.empty(n, dtype=torch.int64, device=dev)
+        _lib.check(lib.bh_trace_record_trajectory_f64(
+            self.params, _lib.ptr(self.soa_in), ld, _lib.ptr(self.soa_out), ld,
+            _lib.ptr(self.status), _lib.ptr(self.steps), _lib.ptr(self.attempts),
+            _lib.ptr(self.iters), _lib.ptr(order), n, _lib.ptr(self.pool), self.pool_pages,
+            _lib.ptr(self.page_link), _lib.ptr(self.last_page), self.score_radius,
+            _lib.ptr(self.term), _lib.ptr(self.prev), _lib.ptr(self.evy1),
+            _lib.ptr(self.ev_step), _lib.ptr(self.theta), _lib.ptr(self.event),
+            _lib.ptr(self.workspace), stream), "trace_record_trajectory")
+
+    @staticmethod
+    def _expected_steps(st, state0, cfg):
+        """Per-ray expected accepted steps (batch.expected_steps), at most
+        max_steps."""
+        import numpy as np
+
+        from .batch import TraceJob, expected_steps
+
+        est = expected_steps(TraceJob(st, state0, cfg.escape_radius, rtol=cfg.rtol,
+                                      max_step=cfg.max_step, lambda_end=cfg.lambda_end))
+        est = np.broadcast_to(np.asarray(est, dtype=np.float64), (state0.shape[0],))
+        return np.minimum(est, max(cfg.max_steps, 0))
+
+    @staticmethod
+    def _pages(steps, n):
+        """Pages holding `steps` accepted steps per ray (at least one in all)."""
+        if not n:
+            return 1
+        if isinstance(steps, torch.Tensor):
+            return max(int(((steps + TRAJ_PAGE_STEPS - 1) // TRAJ_PAGE_STEPS).sum().item()), 1)
+        import numpy as np
+
+        return max(int(np.ceil(np.asarray(steps) / TRAJ_PAGE_STEPS).sum()), 1)
+
+    @staticmethod
+    def _store_fits(pages, dev):
+        need = pages * (TRAJ_PAGE_STEPS * TRAJ_VALUES * 8 + 8)
+        return need <= TRAJ_BUDGET_BYTES and need <= torch.cuda.mem_get_info(dev)[0] // 2
+
+    def stored_steps(self, i: int) -> torch.Tensor:
+        """Ray i's stored steps (accepted steps, 9) from a trajectory store:
+        start state and step size, in order (for checks; walks the page
+        chain on the host)."""
+        if self.last_page is None:
+            raise ValueError("the recording has no trajectory store")
+        total = int(self.steps[i].item())
+        link = self.page_link.cpu()
+        pages = []
+        p = int(self.last_page[i].item())
+        while p >= 0:
+            pages.append(p)
+            p = int(link[p])
+        pages.reverse()
+        if len(pages) != -(-total // TRAJ_PAGE_STEPS):
+            raise RuntimeError("page chain does not match the step count")
+        pool = self.pool.view(-1, TRAJ_PAGE_STEPS, TRAJ_VALUES)
+        idx = torch.tensor(pages, dtype=torch.int64, device=self.pool.device)
+        return pool[idx].reshape(-1, TRAJ_VALUES)[:total].clone()
 
     @staticmethod
     def _pre_pass(st, state0, cfg):
@@ -204,9 +276,10 @@ class Recorded:
         # longest reversals first, rays of similar length in one wavefront
         order = torch.argsort(self.ev_step, descending=True) if self.cfg.sort_backward and n \
             else None
-        if self.traj is not None:
+        if self.last_page is not None:
             _lib.check(lib.bh_trace_adjoint_trajectory_f64(
-                self.params, n, self.score_radius, _lib.ptr(self.traj_off), _lib.ptr(self.traj),
+                self.params, n, self.score_radius, _lib.ptr(self.pool), self.pool_pages,
+                _lib.ptr(self.page_link), _lib.ptr(self.last_page), _lib.ptr(self.steps),
                 _lib.ptr(self.event), _lib.ptr(self.ev_step), _lib.ptr(self.evy1),
                 _lib.ptr(self.prev), _lib.ptr(self.theta), _lib.ptr(gterm), _lib.ptr(gy0),
                 _lib.ptr(gspin), _lib.ptr(order), _lib.stream_handle()), "trace_adjoint_trajectory")

@@ -248,15 +248,23 @@ static int32_t check_adjoint_args(const bh_trace_params *p, int64_t n, int32_t K
   return BH_OK;
 }
 
+// The trajectory store a reverse pass reads (last_page NULL: checkpoints).
+struct TrajIn {
+  const double *pool;
+  const int64_t *page_link, *last_page, *steps;
+  int64_t pool_pages;
+};
+
 static int32_t launch_adjoint(const bh_trace_params *p, int64_t n, int32_t K, double score_radius,
                               const double *ckpt, const int32_t *event, const int64_t *ev_step,
                               const double *evy1, const double *prev, const double *theta,
                               const double *gterm, double *gy0, double *gspin, double *scratch,
-                              const int64_t *order, const double *traj, const int64_t *traj_off,
-                              void *stream, const char *name) {
+                              const int64_t *order, const TrajIn &traj, void *stream,
+                              const char *name) {
   JobConst c = make_job_const(*p);
-  AdjointIO a{ckpt,  event,   ev_step, evy1, prev,     theta,        gterm, gy0,
-              gspin, scratch, order,   traj, traj_off, score_radius, n,     K};
+  AdjointIO a{ckpt,    event,    ev_step,    evy1,  prev,       theta,        gterm,
+              gy0,     gspin,    scratch,    order, traj.pool, traj.page_link,
+              traj.last_page, traj.steps, traj.pool_pages, score_radius, n, K};
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = p->mode == BH_MODE_PRECISE ? precise::launch_trace_adjoint(c, a, s)
                                             : fast::launch_trace_adjoint(c, a, s);
@@ -309,6 +317,7 @@ static int32_t launch_record(const bh_trace_params *p, const TraceIO &io, const 
   JobConst c = make_job_const(*p);
   hipError_t e = hipSuccess;
   if (iterations) e = launch_zero_u64((unsigned long long *)iterations, s);
+  if (e == hipSuccess && rec.last_page) e = launch_zero_u64(rec.pages_taken, s);
   int cus = 0;
   if (e == hipSuccess && n > 0 && p->mode == BH_MODE_PRECISE) e = device_cus(&cus);
   if (e == hipSuccess && n > 0) {
@@ -341,27 +350,38 @@ int32_t bh_trace_record_f64(const bh_trace_params *p, const double *state_in_soa
     return fail(BH_EINVAL, "NULL buffer");
   TraceIO io{state_in_soa, ld_in, state_out_soa, ld_out, status, steps, attempts,
              (unsigned long long *)iterations, order, n};
-  RecordIO rec{ckpt,    n_ckpt,  term,         prev, evy1, ev_step, theta,
-               event,   nullptr, nullptr,      score_radius, cap,     K};
+  RecordIO rec{ckpt,  n_ckpt,  term,    prev,    evy1,         ev_step, theta, event,
+               nullptr, nullptr, nullptr, nullptr, 0,       score_radius, cap,   K};
   return launch_record(p, io, rec, workspace, stream, "bh_trace_record_f64");
 }
+
+// Pool pages beyond this are refused (page * BH_TRAJ_PAGE_STEPS * 9 must
+// fit int64 with room to spare).
+constexpr int64_t MAX_POOL_PAGES = (int64_t)1 << 48;
 
 int32_t bh_trace_record_trajectory_f64(
     const bh_trace_params *p, const double *state_in_soa, int64_t ld_in, double *state_out_soa,
     int64_t ld_out, int32_t *status, int64_t *steps, int64_t *attempts, uint64_t *iterations,
-    const int64_t *order, int64_t n, const int64_t *traj_off, double *traj, double score_radius,
-    double *term, double *prev, double *evy1, int64_t *ev_step, double *theta, int32_t *event,
-    uint64_t *workspace, void *stream) {
+    const int64_t *order, int64_t n, double *pool, int64_t pool_pages, int64_t *page_link,
+    int64_t *last_page, double score_radius, double *term, double *prev, double *evy1,
+    int64_t *ev_step, double *theta, int32_t *event, uint64_t *workspace, void *stream) {
   int32_t rc = check_params(p);
   if (rc) return rc;
   if (!rays_ok(n) || ld_in < n || ld_out < n) return fail(BH_EINVAL, "bad n / ld");
-  if (n > 0 && (!state_in_soa || !state_out_soa || !status || !steps || !attempts || !traj_off ||
-                !traj || !term || !prev || !evy1 || !ev_step || !theta || !event || !workspace))
+  if (pool_pages < 0 || pool_pages > MAX_POOL_PAGES)
+    return fail(BH_EINVAL, "pool_pages out of range [0, 2^48]");
+  if (n > 0 && (!state_in_soa || !state_out_soa || !status || !steps || !attempts ||
+                !last_page || !term || !prev || !evy1 || !ev_step || !theta || !event ||
+                !workspace || (pool_pages > 0 && (!pool || !page_link))))
     return fail(BH_EINVAL, "NULL buffer");
   TraceIO io{state_in_soa, ld_in, state_out_soa, ld_out, status, steps, attempts,
              (unsigned long long *)iterations, order, n};
-  RecordIO rec{nullptr, nullptr, term,     prev,         evy1, ev_step, theta,
-               event,   traj,    traj_off, score_radius, 0,    1};
+  // last_page marks the recording as a trajectory store (with a pool of
+  // zero pages every ray with a step overflows)
+  RecordIO rec{nullptr, nullptr,    term,      prev,      evy1,
+               ev_step, theta,      event,     pool,      page_link,
+               last_page, (unsigned long long *)workspace + 1, pool_pages, score_radius, 0,
+               1};
   return launch_record(p, io, rec, workspace, stream, "bh_trace_record_trajectory_f64");
 }
 
@@ -376,7 +396,7 @@ int32_t bh_trace_adjoint_f64(const bh_trace_params *p, int64_t n, int32_t K, dou
   if (rc) return rc;
   if (n > 0 && !scratch) return fail(BH_EINVAL, "NULL buffer");
   return launch_adjoint(p, n, K, score_radius, ckpt, event, ev_step, evy1, prev, theta, gterm,
-                        gy0, gspin, scratch, order, nullptr, nullptr, stream,
+                        gy0, gspin, scratch, order, TrajIn{}, stream,
                         "bh_trace_adjoint_f64");
 }
 
@@ -401,25 +421,29 @@ int32_t bh_trace_adjoint_static_f64(const bh_trace_params *p, int64_t n, int32_t
   rc = static_scratch(&ptr);
   if (rc) return rc;
   return launch_adjoint(p, n, K, score_radius, ckpt, event, ev_step, evy1, prev, theta, gterm,
-                        gy0, gspin, ptr, order, nullptr, nullptr, stream,
+                        gy0, gspin, ptr, order, TrajIn{}, stream,
                         "bh_trace_adjoint_static_f64");
 }
 
 int32_t bh_trace_adjoint_trajectory_f64(const bh_trace_params *p, int64_t n,
-                                        double score_radius, const int64_t *traj_off,
-                                        const double *traj, const int32_t *event,
-                                        const int64_t *ev_step, const double *evy1,
-                                        const double *prev, const double *theta,
-                                        const double *gterm, double *gy0, double *gspin,
-                                        const int64_t *order, void *stream) {
+                                        double score_radius, const double *pool,
+                                        int64_t pool_pages, const int64_t *page_link,
+                                        const int64_t *last_page, const int64_t *steps,
+                                        const int32_t *event, const int64_t *ev_step,
+                                        const double *evy1, const double *prev,
+                                        const double *theta, const double *gterm, double *gy0,
+                                        double *gspin, const int64_t *order, void *stream) {
   int32_t rc = check_params(p);
   if (rc) return rc;
   if (!rays_ok(n)) return fail(BH_EINVAL, "bad n");
-  if (n > 0 && (!traj_off || !traj || !event || !ev_step || !evy1 || !prev || !theta || !gterm ||
-                !gy0 || !gspin))
+  if (pool_pages < 0 || pool_pages > MAX_POOL_PAGES)
+    return fail(BH_EINVAL, "pool_pages out of range [0, 2^48]");
+  if (n > 0 && (!last_page || !steps || !event || !ev_step || !evy1 || !prev || !theta ||
+                !gterm || !gy0 || !gspin || (pool_pages > 0 && (!pool || !page_link))))
     return fail(BH_EINVAL, "NULL buffer");
+  const TrajIn traj{pool, page_link, last_page, steps, pool_pages};
   return launch_adjoint(p, n, 1, score_radius, nullptr, event, ev_step, evy1, prev, theta, gterm,
-                        gy0, gspin, nullptr, order, traj, traj_off, stream,
+                        gy0, gspin, nullptr, order, traj, stream,
                         "bh_trace_adjoint_trajectory_f64");
 }
 

@@ -73,11 +73,17 @@ struct RecordIO {
   int64_t *ev_step;  // [n]: accepted-step index at the end of the event step
   double *theta;     // [n]: dense-output parameter of the event in that step
   int32_t *event;    // [n]: 0 none, 1 escape sphere, 2 scoring sphere, -1 checkpoint overflow
-  // optional trajectory (instead of checkpoints): ray i's accepted step s
-  // (start state and step size, TRAJ_VALUES doubles) at
-  // traj[(traj_off[i] + s) * TRAJ_VALUES], s < traj_off[i + 1] - traj_off[i]
-  double *traj;
-  const int64_t *traj_off;
+  // optional trajectory store (instead of checkpoints): pages of
+  // TRAJ_PAGE_STEPS accepted steps taken from a pool by an atomic counter;
+  // step j of page p (start state and step size, TRAJ_VALUES doubles) at
+  // pool[(p * TRAJ_PAGE_STEPS + j) * TRAJ_VALUES]; page_link[p] is the
+  // ray's previous page (-1 for its first), last_page[i] ray i's last page;
+  // last_page != NULL selects the trajectory store
+  double *pool;
+  int64_t *page_link;
+  int64_t *last_page;
+  unsigned long long *pages_taken;  // pool counter (pages requested)
+  int64_t pool_pages;
   double score_radius;
   int32_t cap;
   int32_t K;
@@ -86,6 +92,8 @@ struct RecordIO {
 // Values per stored trajectory step: the step's start state (8) and its
 // accepted step size.
 constexpr int TRAJ_VALUES = 9;
+constexpr int TRAJ_PAGE_STEPS = BH_TRAJ_PAGE_STEPS;
+static_assert((TRAJ_PAGE_STEPS & (TRAJ_PAGE_STEPS - 1)) == 0, "page size: a power of two");
 
 struct AdjointIO {
   const double *ckpt;
@@ -99,8 +107,13 @@ struct AdjointIO {
   double *gspin;        // [n] per-ray spin adjoint
   double *scratch;      // [ceil(n / 64)][9 K][64]
   const int64_t *order; // optional [n]: lane slot -> ray
-  const double *traj;   // optional trajectory (RecordIO::traj): no replay
-  const int64_t *traj_off;
+  // optional trajectory store (RecordIO's pool, page_link, last_page) and
+  // each ray's accepted-step count: no replay
+  const double *pool;
+  const int64_t *page_link;
+  const int64_t *last_page;
+  const int64_t *steps;
+  int64_t pool_pages;
   double score_radius;
   int64_t n;
   int32_t K;

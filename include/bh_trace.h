@@ -197,21 +197,29 @@ int32_t bh_trace_record_f64(const bh_trace_params *p, const double *state_in_soa
                             int64_t *ev_step, double *theta, int32_t *event, uint64_t *workspace,
                             void *stream);
 
+/* Accepted steps per page of a trajectory store. */
+#define BH_TRAJ_PAGE_STEPS 32
+
 /* bh_trace_record_f64 storing every accepted step instead of checkpoints,
- * for bh_trace_adjoint_trajectory_f64 (no replay in the reverse pass).
- * traj_off: device int64[n + 1], non-decreasing; ray i's steps go to
- * traj[(traj_off[i] + s) * 9 + 0..8] (start state of accepted step s, then
- * its step size), s < traj_off[i + 1] - traj_off[i]; traj holds
- * traj_off[n] * 9 device doubles.  A ray with more accepted steps than its
- * slots gets event -1.  Sizing the slots from a plain trace's per-ray step
- * counts is exact (the recording takes the same steps).  Every other output
- * is bh_trace_record_f64's, bit for bit. */
+ * for bh_trace_adjoint_trajectory_f64 (no replay in the reverse pass).  The
+ * store is a pool of pool_pages pages of BH_TRAJ_PAGE_STEPS steps (pool:
+ * pool_pages * BH_TRAJ_PAGE_STEPS * 9 device doubles); a ray takes a page
+ * from the pool, through an atomic counter, when its accepted step s is a
+ * multiple of BH_TRAJ_PAGE_STEPS, and writes step s (start state, then the
+ * accepted step size) at pool[(page * BH_TRAJ_PAGE_STEPS + s mod
+ * BH_TRAJ_PAGE_STEPS) * 9 + 0..8].  page_link [pool_pages]: each taken
+ * page's predecessor of the same ray (-1 for its first page); last_page [n]:
+ * each ray's last page (-1 without steps).  workspace: device uint64[2]
+ * (the quad form's ray queue, the pool counter); after the call
+ * workspace[1] holds the pages requested: above pool_pages, the rays that
+ * found the pool empty stopped storing and have event -1.  Every output
+ * other than the store is bh_trace_record_f64's, bit for bit. */
 int32_t bh_trace_record_trajectory_f64(
     const bh_trace_params *p, const double *state_in_soa, int64_t ld_in, double *state_out_soa,
     int64_t ld_out, int32_t *status, int64_t *steps, int64_t *attempts, uint64_t *iterations,
-    const int64_t *order, int64_t n, const int64_t *traj_off, double *traj, double score_radius,
-    double *term, double *prev, double *evy1, int64_t *ev_step, double *theta, int32_t *event,
-    uint64_t *workspace, void *stream);
+    const int64_t *order, int64_t n, double *pool, int64_t pool_pages, int64_t *page_link,
+    int64_t *last_page, double score_radius, double *term, double *prev, double *evy1,
+    int64_t *ev_step, double *theta, int32_t *event, uint64_t *workspace, void *stream);
 
 /* Reverse pass: given gterm [8][n] = dL/d(terminal point), writes
  * gy0 [8][n] = dL/d(initial state) and gspin [n] = per-ray dL/d(spin),
@@ -252,19 +260,21 @@ int32_t bh_trace_adjoint_static_f64(const bh_trace_params *p, int64_t n, int32_t
                                     const double *theta, const double *gterm, double *gy0,
                                     double *gspin, const int64_t *order, void *stream);
 
-/* bh_trace_adjoint_f64 over a trajectory stored by
- * bh_trace_record_trajectory_f64 (same traj_off and traj): each step's
- * start state and size are read instead of replayed, so there is no
- * scratch and no checkpoint interval.  Results equal bh_trace_adjoint_f64's
- * on a checkpoint recording of the same rays, bit for bit.  Rays with event
- * -1, or fewer stored steps than ev_step, get NaN. */
+/* bh_trace_adjoint_f64 over a trajectory store written by
+ * bh_trace_record_trajectory_f64 (the same pool, pool_pages, page_link and
+ * last_page, and its steps output): each step's start state and size are
+ * read instead of replayed, so there is no scratch and no checkpoint
+ * interval.  Results equal bh_trace_adjoint_f64's on a checkpoint recording
+ * of the same rays, bit for bit.  Rays with event -1, or whose page chain
+ * leaves [0, pool_pages) before their event step, get NaN. */
 int32_t bh_trace_adjoint_trajectory_f64(const bh_trace_params *p, int64_t n,
-                                        double score_radius, const int64_t *traj_off,
-                                        const double *traj, const int32_t *event,
-                                        const int64_t *ev_step, const double *evy1,
-                                        const double *prev, const double *theta,
-                                        const double *gterm, double *gy0, double *gspin,
-                                        const int64_t *order, void *stream);
+                                        double score_radius, const double *pool,
+                                        int64_t pool_pages, const int64_t *page_link,
+                                        const int64_t *last_page, const int64_t *steps,
+                                        const int32_t *event, const int64_t *ev_step,
+                                        const double *evy1, const double *prev,
+                                        const double *theta, const double *gterm, double *gy0,
+                                        double *gspin, const int64_t *order, void *stream);
 
 /* Vector-Jacobian product of geodesic_rhs (geodesics.py:42-74) at n states
  * (rows): zbar = J^T v (n, 8) and abar = v . d f / d spin (n). */

@@ -282,32 +282,47 @@ std::vector<Case> build_cases() {
   add("adjoint: n INT64_MAX", [=] { return adjoint(BIG, 8, d); }, "bad n");
   add("adjoint: K INT32_MAX", [=] { return adjoint(4, INT32_MAX, d); }, "1 <= K <= 2^20");
   add("adjoint: scratch NULL", [=] { return adjoint(4, 8, nullptr); }, "NULL buffer");
-  auto record_traj = [=](int64_t n, const int64_t *off, double *traj) {
+  auto record_traj = [=](int64_t n, double *pool, int64_t pages, int64_t *link, int64_t *last) {
     bh_trace_params p = good_params();
-    return bh_trace_record_trajectory_f64(&p, d, 4, d, 4, i32, i64, i64, u64, nullptr, n, off,
-                                          traj, 20.0, d, d, d, i64, d, i32, u64, nullptr);
+    return bh_trace_record_trajectory_f64(&p, d, 4, d, 4, i32, i64, i64, u64, nullptr, n, pool,
+                                          pages, link, last, 20.0, d, d, d, i64, d, i32, u64,
+                                          nullptr);
   };
-  add("record trajectory: n -1", [=] { return record_traj(-1, i64, d); }, "bad n / ld");
-  add("record trajectory: n > ld", [=] { return record_traj(5, i64, d); }, "bad n / ld");
-  add("record trajectory: offsets NULL", [=] { return record_traj(4, nullptr, d); },
+  add("record trajectory: n -1", [=] { return record_traj(-1, d, 2, i64, i64); }, "bad n / ld");
+  add("record trajectory: n > ld", [=] { return record_traj(5, d, 2, i64, i64); },
+      "bad n / ld");
+  add("record trajectory: pages -1", [=] { return record_traj(4, d, -1, i64, i64); },
+      "pool_pages out of range");
+  add("record trajectory: pages 2^48+1", [=] {
+    return record_traj(4, d, ((int64_t)1 << 48) + 1, i64, i64);
+  }, "pool_pages out of range");
+  add("record trajectory: pool NULL", [=] { return record_traj(4, nullptr, 2, i64, i64); },
       "NULL buffer");
-  add("record trajectory: store NULL", [=] { return record_traj(4, i64, nullptr); },
+  add("record trajectory: page_link NULL", [=] { return record_traj(4, d, 2, nullptr, i64); },
       "NULL buffer");
-  auto adjoint_traj = [=](int64_t n, const int64_t *off, const double *traj) {
+  add("record trajectory: last_page NULL", [=] { return record_traj(4, d, 0, i64, nullptr); },
+      "NULL buffer");
+  auto adjoint_traj = [=](int64_t n, const double *pool, int64_t pages, const int64_t *last,
+                          const int64_t *steps) {
     bh_trace_params p = good_params();
-    return bh_trace_adjoint_trajectory_f64(&p, n, 20.0, off, traj, i32, i64, d, d, d, d, d, d,
-                                           nullptr, nullptr);
+    return bh_trace_adjoint_trajectory_f64(&p, n, 20.0, pool, pages, i64, last, steps, i32, i64,
+                                           d, d, d, d, d, d, nullptr, nullptr);
   };
-  add("adjoint trajectory: n INT64_MAX", [=] { return adjoint_traj(BIG, i64, d); }, "bad n");
-  add("adjoint trajectory: offsets NULL", [=] { return adjoint_traj(4, nullptr, d); },
+  add("adjoint trajectory: n INT64_MAX", [=] { return adjoint_traj(BIG, d, 2, i64, i64); },
+      "bad n");
+  add("adjoint trajectory: pages -5", [=] { return adjoint_traj(4, d, -5, i64, i64); },
+      "pool_pages out of range");
+  add("adjoint trajectory: pool NULL", [=] { return adjoint_traj(4, nullptr, 2, i64, i64); },
       "NULL buffer");
-  add("adjoint trajectory: store NULL", [=] { return adjoint_traj(4, i64, nullptr); },
+  add("adjoint trajectory: last_page NULL", [=] { return adjoint_traj(4, d, 2, nullptr, i64); },
+      "NULL buffer");
+  add("adjoint trajectory: steps NULL", [=] { return adjoint_traj(4, d, 2, i64, nullptr); },
       "NULL buffer");
   add("adjoint trajectory: mode 3", [=] {
     bh_trace_params p = good_params();
     p.mode = 3;
-    return bh_trace_adjoint_trajectory_f64(&p, 4, 20.0, i64, d, i32, i64, d, d, d, d, d, d,
-                                           nullptr, nullptr);
+    return bh_trace_adjoint_trajectory_f64(&p, 4, 20.0, d, 2, i64, i64, i64, i32, i64, d, d, d, d,
+                                           d, d, nullptr, nullptr);
   }, "mode must be");
   add("rhs_vjp: spin NaN", [=] {
     return bh_rhs_vjp_f64(1.0, __builtin_nan(""), d, d, 4, d, d, nullptr);

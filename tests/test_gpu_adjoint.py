@@ -310,7 +310,7 @@ def test_checkpoint_interval_does_not_change_gradients(hip):
     out = []
     for K in (1, 5, 32, 4096):
         sg = torch.from_numpy(s0).cuda().requires_grad_(True)
-        Y, _ = adjoint.trace_terminal(sg, SPIN, _settings(K=K))
+        Y, _ = adjoint.trace_terminal(sg, SPIN, _settings(K=K, trajectory=False))
         (Y * gY).sum().backward()
         out.append(sg.grad.cpu().numpy())
     for o in out[1:]:
@@ -336,20 +336,22 @@ def test_backward_lane_order_does_not_change_gradients(hip):
     assert out[0][1] == out[1][1]
 
 
+@pytest.mark.parametrize("trajectory", [False, True])
 @pytest.mark.parametrize("mode", ["precise", "fast"])
-def test_backward_stage_layouts_give_the_same_bits(hip, mode):
+def test_backward_stage_layouts_give_the_same_bits(hip, mode, trajectory):
     """The reverse pass keeps a step's stages in registers when the launch
     has at most one wavefront per SIMD (a few long rays) and in LDS
     otherwise; both run the same explicit FMAs.  Every 101st ray of a
     70,000-ray camera (LDS layout) reversed alone (register layout): per-ray
-    initial-state and spin gradients equal bit for bit."""
+    initial-state and spin gradients equal bit for bit, replaying
+    checkpoints or reading a trajectory store."""
     from blackhorizon_amd import adjoint, _lib
 
     props = torch.cuda.get_device_properties(0)
     big = 280 * 250
     assert -(-big // 64) > props.multi_processor_count * 4  # LDS layout
     s0 = adjoint.camera_state(SPIN, INC, DIST, FOV, 280, 250).detach()
-    cfg = _settings(mode)
+    cfg = _settings(mode, trajectory=trajectory)
     sub = torch.arange(0, big, 101, device="cuda")
     g = torch.from_numpy(np.random.default_rng(9).normal(size=(big, 8))).cuda()
     rec = adjoint.Recorded(s0, SPIN, cfg)
@@ -378,7 +380,7 @@ def test_recording_on_quads_gives_the_same_bits(hip):
     big = 280 * 250
     assert -(-big // 64) > props.multi_processor_count * 4  # one lane per ray
     s0 = adjoint.camera_state(SPIN, INC, DIST, FOV, 280, 250).detach()
-    cfg = _settings("precise", cap=400)
+    cfg = _settings("precise", cap=400, trajectory=False)
     sub = torch.arange(0, big, 101, device="cuda")
     a = adjoint.Recorded(s0, SPIN, cfg)
     b = adjoint.Recorded(s0[sub].contiguous(), SPIN, cfg)
@@ -442,23 +444,24 @@ def _bits(t):
 @pytest.mark.parametrize("mode,width,height", [("precise", 64, 48), ("precise", 280, 250),
                                                ("fast", 64, 48)])
 def test_trajectory_store_gives_the_same_bits(hip, mode, width, height):
-    """A recording that stores every accepted step (trajectory store, no
-    checkpoints) and the backward that reverses the stored steps without a
-    replay give what the checkpoint recording and its replaying backward
+    """A recording that stores every accepted step (paged trajectory store,
+    no checkpoints) and the backward that reverses the stored steps without
+    a replay give what the checkpoint recording and its replaying backward
     give, bit for bit: final states, statuses, step and attempt counts,
     event steps, terminal points, theta, event kinds, and the gradients of
     the initial states and the spin.  Camera rays escape and are captured
     (both event kinds); 3,072 precise rays record on quads, 70,000 on one
-    lane per ray.  Every checkpoint equals the stored start state of its
-    step, and the stored step sizes sum to the affine length of each ray's
-    steps."""
+    lane per ray.  The pool counter equals the pages the rays' step counts
+    need, each ray's page chain holds its steps in order (its first stored
+    state is its initial state), and every checkpoint equals the stored
+    start state of its step."""
     from blackhorizon_amd import adjoint
 
     s0 = adjoint.camera_state(SPIN, INC, DIST, FOV, width, height).detach()
     n = s0.shape[0]
     a = adjoint.Recorded(s0, SPIN, _settings(mode, cap=400, trajectory=False))
-    b = adjoint.Recorded(s0, SPIN, _settings(mode, cap=400, trajectory=True))
-    assert a.traj is None and b.traj is not None and b.ckpt is None
+    b = adjoint.Recorded(s0, SPIN, _settings(mode, trajectory=True))
+    assert a.last_page is None and b.last_page is not None and b.ckpt is None
     for name in ("soa_out", "term", "prev", "evy1"):
         np.testing.assert_array_equal(_bits(getattr(a, name)[:, :n]),
                                       _bits(getattr(b, name)[:, :n]), err_msg=name)
@@ -467,18 +470,18 @@ def test_trajectory_store_gives_the_same_bits(hip, mode, width, height):
                                       err_msg=name)
     ev = b.event.cpu().numpy()
     assert (ev == 1).any() and (ev == 2).any() and (ev >= 0).all()
-    off = b.traj_off.cpu().numpy()
     steps = b.steps.cpu().numpy()
-    np.testing.assert_array_equal(np.diff(off), steps)
-    traj = b.traj.cpu().numpy()[: off[-1] * adjoint.TRAJ_VALUES].reshape(-1, 9)
+    P = adjoint.TRAJ_PAGE_STEPS
+    assert int(b.workspace[1]) == int(np.sum(-(-steps // P))) <= b.pool_pages
     ck, nck = a.ckpt.cpu().numpy(), a.n_ckpt.cpu().numpy()
     K = a.cfg.K
-    for k in range(0, n, 37):
+    for k in range(0, n, 97):
+        t = b.stored_steps(k).cpu().numpy()
+        assert t.shape == (steps[k], 9)
+        np.testing.assert_array_equal(t[0, :8].view(np.int64), _bits(s0[k]))
         for j in range(1, nck[k]):  # checkpoint j: the state after j K steps
             np.testing.assert_array_equal(ck[j, :8, k].view(np.int64),
-                                          traj[off[k] + j * K, :8].view(np.int64))
-        np.testing.assert_array_equal(traj[off[k], :8].view(np.int64),
-                                      _bits(s0[k]).view(np.int64))
+                                          t[j * K, :8].view(np.int64))
     gY = torch.from_numpy(np.random.default_rng(3).normal(size=(n, 8))).cuda()
     ga, sa = a.backward(gY)
     gb, sb = b.backward(gY)
@@ -487,64 +490,89 @@ def test_trajectory_store_gives_the_same_bits(hip, mode, width, height):
     np.testing.assert_array_equal(_bits(sa), _bits(sb))
 
 
-def test_trajectory_slots_too_few(hip):
-    """A ray given fewer trajectory slots than it takes steps is recorded
-    as event -1 (nothing written past its slots); the backward gives NaN to
-    a ray whose slots are fewer than its event step, the other rays their
-    gradients."""
+def test_trajectory_store_on_quads_equals_one_lane(hip):
+    """Every 101st ray of a 70,000-ray precise camera recorded alone (quad
+    recording) stores the same steps, bit for bit, as in the whole camera's
+    recording (one lane per ray), though on other pages."""
+    from blackhorizon_amd import adjoint
+
+    s0 = adjoint.camera_state(SPIN, INC, DIST, FOV, 280, 250).detach()
+    sub = torch.arange(0, s0.shape[0], 101, device="cuda")
+    cfg = _settings("precise", trajectory=True)
+    a = adjoint.Recorded(s0, SPIN, cfg)
+    b = adjoint.Recorded(s0[sub].contiguous(), SPIN, cfg)
+    for k in range(0, sub.shape[0], 7):
+        np.testing.assert_array_equal(_bits(a.stored_steps(int(sub[k]))),
+                                      _bits(b.stored_steps(k)), err_msg=f"ray {k}")
+
+
+def test_trajectory_pool_too_small(hip, monkeypatch):
+    """A pool that runs out: the rays that find it empty stop storing and
+    get event -1, nothing lands past the pool, and the pool counter reports
+    more pages requested than there were.  Recorded then sizes the pool from
+    a plain trace's step counts and records again: the same results and
+    gradients as the checkpoint path.  The reverse pass gives NaN to a ray
+    whose page chain is cut, the others their gradients."""
     from blackhorizon_amd import _lib, adjoint
 
     s0 = adjoint.camera_state(SPIN, INC, DIST, FOV, 16, 12).detach()
     n = s0.shape[0]
-    b = adjoint.Recorded(s0, SPIN, _settings("precise", cap=400, trajectory=True))
+    ref = adjoint.Recorded(s0, SPIN, _settings("precise", trajectory=False))
+    need = int(np.sum(-(-ref.steps.cpu().numpy() // adjoint.TRAJ_PAGE_STEPS)))
     lib = _lib.require_device()
-    steps = b.steps.clone()
-    assert int(steps[5]) > 2
-    steps[5] -= 1
-    off = torch.zeros(n + 1, dtype=torch.int64, device="cuda")
-    torch.cumsum(steps, 0, out=off[1:])
-    traj = torch.full_like(b.traj, 7.0)
-    ws = torch.zeros(1, dtype=torch.int64, device="cuda")
-    out = {k: torch.empty_like(getattr(b, k)) for k in ("soa_out", "status", "steps", "attempts",
-                                                        "iters", "term", "prev", "evy1",
-                                                        "ev_step", "theta", "event")}
-    ld = n
+    short = need // 2
+    guard = 4  # pages past the pool, filled with a sentinel
+    pool = torch.full(((short + guard) * adjoint.TRAJ_PAGE_STEPS * 9,), 7.0,
+                      dtype=torch.float64, device="cuda")
+    link = torch.full((short + guard,), -7, dtype=torch.int64, device="cuda")
+    last = torch.empty(n, dtype=torch.int64, device="cuda")
+    ws = torch.zeros(2, dtype=torch.int64, device="cuda")
+    out = {k: torch.empty_like(getattr(ref, k)) for k in ("soa_out", "status", "steps",
+                                                          "attempts", "iters", "term", "prev",
+                                                          "evy1", "ev_step", "theta", "event")}
     _lib.check(lib.bh_trace_record_trajectory_f64(
-        b.params, _lib.ptr(b.soa_in), ld, _lib.ptr(out["soa_out"]), ld, _lib.ptr(out["status"]),
+        ref.params, _lib.ptr(ref.soa_in), n, _lib.ptr(out["soa_out"]), n, _lib.ptr(out["status"]),
         _lib.ptr(out["steps"]), _lib.ptr(out["attempts"]), _lib.ptr(out["iters"]), None, n,
-        _lib.ptr(off), _lib.ptr(traj), b.score_radius, _lib.ptr(out["term"]),
-        _lib.ptr(out["prev"]), _lib.ptr(out["evy1"]), _lib.ptr(out["ev_step"]),
-        _lib.ptr(out["theta"]), _lib.ptr(out["event"]), _lib.ptr(ws), _lib.stream_handle()),
-        "record")
+        _lib.ptr(pool), short, _lib.ptr(link), _lib.ptr(last), ref.score_radius,
+        _lib.ptr(out["term"]), _lib.ptr(out["prev"]), _lib.ptr(out["evy1"]),
+        _lib.ptr(out["ev_step"]), _lib.ptr(out["theta"]), _lib.ptr(out["event"]), _lib.ptr(ws),
+        _lib.stream_handle()), "record")
     ev = out["event"].cpu().numpy()
-    want = b.event.cpu().numpy().copy()
-    want[5] = -1
-    np.testing.assert_array_equal(ev, want)
-    # nothing lands past ray 5's slots: ray 6's steps are the full recording's,
-    # and the store's last step (no ray's slot now) is untouched
-    o, ob = off.cpu().numpy(), b.traj_off.cpu().numpy()
-    np.testing.assert_array_equal(_bits(traj[o[6] * 9: o[7] * 9]),
-                                  _bits(b.traj[ob[6] * 9: ob[7] * 9]))
-    assert float(traj[o[-1] * 9:].min()) == 7.0 == float(traj[o[-1] * 9:].max())
-    # the backward on the full recording with ray 5's slots cut below its event step
-    ev_step = b.ev_step.cpu().numpy()
-    assert ev_step[5] >= 2
-    off2 = b.traj_off.clone()
-    off2[5] = off2[6] - int(ev_step[5]) + 1  # ray 5 keeps ev_step - 1 slots, ray 4 gains
+    assert int(ws[1]) > short and (ev == -1).any() and (ev != -1).any()
+    np.testing.assert_array_equal(ev[ev != -1], ref.event.cpu().numpy()[ev != -1])
+    np.testing.assert_array_equal(out["steps"].cpu().numpy(), ref.steps.cpu().numpy())
+    tail = pool[short * adjoint.TRAJ_PAGE_STEPS * 9:]
+    assert float(tail.min()) == 7.0 == float(tail.max())
+    assert (link[short:] == -7).all()
+    # Recorded with an estimate that falls short sizes the pool exactly and records again
+    monkeypatch.setattr(adjoint.Recorded, "_expected_steps",
+                        staticmethod(lambda st, s, cfg: np.ones(s.shape[0])))
+    b = adjoint.Recorded(s0, SPIN, _settings("precise", trajectory=True))
+    assert b.pool_pages == need == int(b.workspace[1])
+    gY = torch.from_numpy(np.random.default_rng(4).normal(size=(n, 8))).cuda()
+    ga, _ = ref.backward(gY)
+    gb, _ = b.backward(gY)
+    np.testing.assert_array_equal(_bits(ga), _bits(gb))
+    np.testing.assert_array_equal(_bits(ref.term[:, :n]), _bits(b.term[:, :n]))
+    # a cut chain: ray 5's last page made -1
+    assert int(b.steps[5]) > 0 and int(b.ev_step[5]) > 0
+    last2 = b.last_page.clone()
+    last2[5] = -1
     gterm = torch.zeros((8, n), dtype=torch.float64, device="cuda")
     gterm[1] = 1.0
     gy0 = torch.empty((8, n), dtype=torch.float64, device="cuda")
     gspin = torch.empty(n, dtype=torch.float64, device="cuda")
     _lib.check(lib.bh_trace_adjoint_trajectory_f64(
-        b.params, n, b.score_radius, _lib.ptr(off2), _lib.ptr(b.traj), _lib.ptr(b.event),
-        _lib.ptr(b.ev_step), _lib.ptr(b.evy1), _lib.ptr(b.prev), _lib.ptr(b.theta),
-        _lib.ptr(gterm), _lib.ptr(gy0), _lib.ptr(gspin), None, _lib.stream_handle()), "adjoint")
+        b.params, n, b.score_radius, _lib.ptr(b.pool), b.pool_pages, _lib.ptr(b.page_link),
+        _lib.ptr(last2), _lib.ptr(b.steps), _lib.ptr(b.event), _lib.ptr(b.ev_step),
+        _lib.ptr(b.evy1), _lib.ptr(b.prev), _lib.ptr(b.theta), _lib.ptr(gterm), _lib.ptr(gy0),
+        _lib.ptr(gspin), None, _lib.stream_handle()), "adjoint")
     g = gy0.cpu().numpy()
-    ref, _ = b.backward(gterm[:, :n].t().contiguous())
-    ref = ref.cpu().numpy()
+    want, _ = b.backward(gterm[:, :n].t().contiguous())
+    want = want.cpu().numpy()
     assert np.isnan(g[:, 5]).all() and np.isnan(float(gspin[5]))
     keep = [k for k in range(n) if k != 5]
-    np.testing.assert_array_equal(g[:, keep].T.view(np.int64), ref[keep].view(np.int64))
+    np.testing.assert_array_equal(g[:, keep].T.view(np.int64), want[keep].view(np.int64))
 
 
 @pytest.mark.parametrize("size,edge_scale,spin_tol", [(96, None, 0.015), (96, 0.05, 0.015),

@@ -524,7 +524,8 @@ def test_timelike_orbit_workload_adjoint_full_size(hip):
     """Gradients of the final positions of all 65,536 workload orbits with
     respect to the spin and every initial state (K = 32): finite for every
     orbit that reaches tau = 1e4; per orbit bitwise equal to a launch of a
-    few of those orbits alone; and for sampled orbits equal to central
+    few of those orbits alone (trajectory store) and to the same orbits
+    recorded with checkpoints; and for sampled orbits equal to central
     differences (h = 1e-6 in p_x, p_y) of the final position to 1e-5."""
     import torch
 
@@ -553,6 +554,14 @@ def test_timelike_orbit_workload_adjoint_full_size(hip):
     Yp, _ = adjoint.trace_terminal(sp, 0.9, cfg)
     (Yp[:, 1:4] * w[pick]).sum().backward()
     assert torch.equal(sp.grad, sg.grad[pick])
+    # the whole workload is recorded into a trajectory store; checkpoints
+    # and their replay give the same bits
+    import dataclasses
+
+    sc = s0[pick].clone().requires_grad_(True)
+    Yc, _ = adjoint.trace_terminal(sc, 0.9, dataclasses.replace(cfg, trajectory=False))
+    (Yc[:, 1:4] * w[pick]).sum().backward()
+    assert torch.equal(sc.grad.view(torch.int64), sg.grad[pick].view(torch.int64))
     h = 1e-6
     for i in range(3):
         for j in (5, 6):

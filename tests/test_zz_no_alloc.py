@@ -62,7 +62,7 @@ def test_static_scratch_adjoint_equals_allocating_entry(hip, mode):
     # and the decreasing-step lane order
     for (w, h), K, sort in (((96, 72), 8, False), ((33, 17), 32, True)):
         cfg = TraceSettings(escape_radius=45.0, rtol=1e-8, atol=1e-10,
-                            capture_margin=0.01 / r_plus, K=K, mode=mode)
+                            capture_margin=0.01 / r_plus, K=K, mode=mode, trajectory=False)
         s0 = _rays(w, h)
         n = s0.shape[0]
         print(f"static scratch adjoint: mode {mode}, {n} rays, K = {K}, sorted lanes {sort}",
