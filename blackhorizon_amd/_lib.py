@@ -114,6 +114,7 @@ SIGNATURES = {
     "bh_trace_adjoint_static_f64": [ctypes.POINTER(TraceParams), _I64, _I32, _D, _P, _P, _P, _P,
                                     _P, _P, _P, _P, _P, _P, _P],
     "bh_rhs_vjp_f64": [_D, _D, _P, _P, _I64, _P, _P, _P],
+    "bh_radial_minimum_f64": [_D, _P, _P, _P, _P, _I64, _I32, _I32, _P, _P, _P],
     "bh_aos_to_soa_f64": [_P, _P, _I64, _I64, _P],
     "bh_soa_to_aos_f64": [_P, _I64, _P, _I64, _P],
     "bh_geometry_f64": [_D, _D, _I32, _P, _I64, _P, _P, _P, _P, _P, _P],

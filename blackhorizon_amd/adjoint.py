@@ -58,9 +58,9 @@ class TraceSettings:
     them -- same gradients, bit for bit.  The pool is sized from the rays'
     expected step counts; should the recording find it too small, the
     exact per-ray counts of a plain trace size it and the recording runs
-    again.  None chooses the store when the estimated pool fits
-    TRAJ_BUDGET_BYTES and half the free device memory; True always; False
-    never (checkpoints)."""
+    again (at the backward: the forward does not wait for the device).  None
+    chooses the store when the estimated pool fits TRAJ_BUDGET_BYTES and a
+    quarter of the device memory; True always; False never (checkpoints)."""
 
     escape_radius: float
     max_steps: int = 20000
@@ -149,13 +149,11 @@ class Recorded:
         self.workspace = torch.zeros(2, dtype=torch.int64, device=dev)
         if pool_pages is not None:
             self.ckpt = self.n_ckpt = None
+            # no host synchronisation here: whether the pool sufficed is
+            # read by the backward (the terminal points, statuses and step
+            # counts are right either way; only the store would be cut)
+            self._redo = (st, state0)
             self._record_store(lib, pool_pages, order, stream)
-            if int(self.workspace[1].item()) > pool_pages:
-                # the estimate fell short: exact sizes from a plain trace
-                pre = self._pre_pass(st, state0, cfg)
-                if cfg.sort_forward:
-                    order = torch.argsort(pre.attempts, descending=True)
-                self._record_store(lib, self._pages(pre.steps, n), order, stream)
             return
         self.ckpt = torch.empty((self.cap, CKPT_VALUES, ld), **f64)
         self.n_ckpt = torch.empty(n, dtype=torch.int32, device=dev)
@@ -215,7 +213,8 @@ class Recorded:
     @staticmethod
     def _store_fits(pages, dev):
         need = pages * (TRAJ_PAGE_STEPS * TRAJ_VALUES * 8 + 8)
-        return need <= TRAJ_BUDGET_BYTES and need <= torch.cuda.mem_get_info(dev)[0] // 2
+        total = torch.cuda.get_device_properties(dev).total_memory
+        return need <= min(TRAJ_BUDGET_BYTES, total // 4)
 
     def stored_steps(self, i: int) -> torch.Tensor:
         """Ray i's stored steps (accepted steps, 9) from a trajectory store:
@@ -267,8 +266,16 @@ class Recorded:
         n = self.n
         ld = max(n, 1)
         dev = self.term.device
+        if self.last_page is not None and int(self.workspace[1].item()) > self.pool_pages:
+            # the estimated pool fell short: size it from a plain trace's
+            # exact step counts and record again (same results)
+            st, state0 = self._redo
+            pre = self._pre_pass(st, state0, self.cfg)
+            order = torch.argsort(pre.attempts, descending=True) if self.cfg.sort_forward \
+                else None
+            self._record_store(lib, self._pages(pre.steps, n), order, _lib.stream_handle())
         if int((self.event < 0).sum().item()) > 0:
-            raise RuntimeError("a ray exceeded its checkpoint capacity or trajectory slots; raise cap")
+            raise RuntimeError("a ray exceeded its checkpoint capacity; raise cap")
         gterm = torch.zeros((8, ld), dtype=torch.float64, device=dev)
         gterm[:, :n] = g_terminal.t()
         gy0 = torch.empty((8, ld), dtype=torch.float64, device=dev)
@@ -464,13 +471,41 @@ def capture_margin(state0: torch.Tensor, spin, mass: float = 1.0, grid: int = 25
     lam = l_z / energy
     eta = q / (energy * energy)
     r_plus = mass + torch.sqrt(mass * mass - a * a)
+    if state0.is_cuda:
+        r_star, at_horizon = _radial_minimum_device(a, mass, lam, eta, r, grid, refine)
+    else:
+        r_star, at_horizon = _radial_minimum_torch(a, mass, lam, eta, r_plus, r, grid, refine)
+    r_eval = torch.where(at_horizon, r_plus.expand_as(r_star), r_star)
+    return _radial_potential(r_eval, a, mass, lam, eta)
+
+
+def _radial_minimum_device(a, mass, lam, eta, r, grid, refine):
+    """capture_margin's detached search by the HIP kernel (one lane per ray;
+    the spin stays on the device)."""
+    lib = _lib.require_device()
+    with torch.no_grad():
+        lamd = lam.detach().contiguous()
+        etad = eta.detach().contiguous()
+        hi = r.detach().contiguous()
+        ad = a.detach().to(device=lamd.device, dtype=torch.float64).reshape(1).contiguous()
+        n = lamd.shape[0]
+        r_star = torch.empty_like(lamd)
+        at_lo = torch.empty(n, dtype=torch.int32, device=lamd.device)
+        _lib.check(lib.bh_radial_minimum_f64(
+            float(mass), _lib.ptr(ad), _lib.ptr(lamd), _lib.ptr(etad), _lib.ptr(hi), n, grid,
+            refine, _lib.ptr(r_star), _lib.ptr(at_lo), _lib.stream_handle()), "radial_minimum")
+    return r_star, at_lo.bool()
+
+
+def _radial_minimum_torch(a, mass, lam, eta, r_plus, r, grid, refine):
+    """capture_margin's detached search in torch operations (host tensors)."""
     with torch.no_grad():
         ad, lamd, etad = a.detach(), lam.detach(), eta.detach()
         lo_r = r_plus.detach().expand_as(r)
         # a state at or inside the horizon has no interval outside it: the
         # search then collapses onto r_+
         hi_r = torch.maximum(r.detach(), lo_r)
-        u = torch.linspace(0.0, 1.0, grid, dtype=torch.float64, device=state0.device)
+        u = torch.linspace(0.0, 1.0, grid, dtype=torch.float64, device=r.device)
         rr = lo_r[:, None] + (hi_r - lo_r)[:, None] * u[None, :]
         vals = _radial_potential(rr, ad, mass, lamd[:, None], etad[:, None])
         k = torch.argmin(vals, dim=1)
@@ -486,8 +521,7 @@ def capture_margin(state0: torch.Tensor, spin, mass: float = 1.0, grid: int = 25
             lo = torch.where(left, lo, c)
         r_star = 0.5 * (lo + hi)
         at_horizon = k == 0
-    r_eval = torch.where(at_horizon, r_plus.expand_as(r_star), r_star)
-    return _radial_potential(r_eval, a, mass, lam, eta)
+    return r_star, at_horizon
 
 
 def edge_weights(state0: torch.Tensor, spin, scale: float, status: torch.Tensor | None = None,
@@ -545,10 +579,13 @@ def shadow_loss(terminal: torch.Tensor, status: torch.Tensor, spin, target_dirs:
     per_ray = torch.where(same, angle_sq(n, target_dirs), zero)
     if state0 is not None and 1 in classes and 2 in classes:
         mism = (((status == 1) & (target_status == 2)) | ((status == 2) & (target_status == 1)))
-        if bool(mism.any()):
-            idx = torch.nonzero(mism)[:, 0]
-            m = capture_margin(state0[idx], spin, mass)
-            per_ray = per_ray.index_add(0, idx, margin_weight * m * m)
+        # every row is scored (no host round trip to pick the mismatched
+        # ones); the others see a fixed regular state, so that no
+        # non-finite margin of a row the loss ignores reaches the gradient
+        regular = torch.tensor([0.0, 10.0, 0.0, 1.0, -1.0, -1.0, 0.1, 0.0],
+                               dtype=state0.dtype, device=state0.device)
+        m = capture_margin(torch.where(mism[:, None], state0, regular), spin, mass)
+        per_ray = per_ray + torch.where(mism, margin_weight * m * m, zero)
     if ray_weights is not None:
         per_ray = per_ray * ray_weights
     return per_ray.sum()

@@ -458,6 +458,20 @@ int32_t bh_rhs_vjp_f64(double mass, double spin, const double *state, const doub
                     "bh_rhs_vjp_f64");
 }
 
+int32_t bh_radial_minimum_f64(double mass, const double *spin, const double *lam,
+                              const double *eta, const double *r_hi, int64_t n, int32_t grid,
+                              int32_t refine, double *r_star, int32_t *at_lo, void *stream) {
+  if (!(mass > 0.0) || !(mass < INFINITY)) return fail(BH_EINVAL, "mass must be positive");
+  if (!rays_ok(n)) return fail(BH_EINVAL, "bad n");
+  if (grid < 2 || grid > (1 << 20) || refine < 0 || refine > 1000)
+    return fail(BH_EINVAL, "need 2 <= grid <= 2^20 and 0 <= refine <= 1000");
+  if (n > 0 && (!spin || !lam || !eta || !r_hi || !r_star || !at_lo))
+    return fail(BH_EINVAL, "NULL buffer");
+  return hip_status(fast::launch_radial_minimum(mass, spin, lam, eta, r_hi, n, grid, refine,
+                                                r_star, at_lo, (hipStream_t)stream),
+                    "bh_radial_minimum_f64");
+}
+
 int32_t bh_aos_to_soa_f64(const double *aos, double *soa, int64_t ld, int64_t n, void *stream) {
   if (!rays_ok(n) || ld < n) return fail(BH_EINVAL, "bad n / ld");
   if (n > 0 && (!aos || !soa)) return fail(BH_EINVAL, "NULL buffer");

@@ -210,6 +210,9 @@ struct LaunchShape {
                                   hipStream_t s);                                  \
   hipError_t launch_rhs_vjp(const JobConst &c, const double *state, const double *v, \
                             int64_t n, double *zbar, double *abar, hipStream_t s); \
+  hipError_t launch_radial_minimum(double mass, const double *spin, const double *lam, \
+                                   const double *eta, const double *r_hi, int64_t n, int grid, \
+                                   int refine, double *r_star, int32_t *at_lo, hipStream_t s); \
   hipError_t launch_integrate_dense(const JobConst &c, const DenseIO &d, hipStream_t s); \
   hipError_t kernel_attributes(int which, int block, int32_t *out);                \
   hipError_t launch_trace_shader(const JobConst &c, const ShaderConst &sp, const ShaderIO &io, \

@@ -281,6 +281,19 @@ int32_t bh_trace_adjoint_trajectory_f64(const bh_trace_params *p, int64_t n,
 int32_t bh_rhs_vjp_f64(double mass, double spin, const double *state, const double *v,
                        int64_t n, double *zbar, double *abar, void *stream);
 
+/* The detached search of the shadow-fit loss's capture margin
+ * (adjoint.capture_margin; no counterpart in the reference): per ray, the
+ * minimiser r_star[i] of the normalised photon radial potential
+ * R(r) / (r^2 + a^2)^2 with constants lam[i] = L_z / E, eta[i] = Q / E^2,
+ * over [r_+, max(r_hi[i], r_+)], r_+ = mass + sqrt(mass^2 - a^2) -- the
+ * first minimum on a grid of `grid` points, then `refine` golden-section
+ * steps on the bracket of its grid neighbours; at_lo[i] = 1 when the grid
+ * minimum is the first point.  spin: device pointer to the spin a (read on
+ * the device, so the caller need not wait for it; |a| > mass gives NaN). */
+int32_t bh_radial_minimum_f64(double mass, const double *spin, const double *lam,
+                              const double *eta, const double *r_hi, int64_t n, int32_t grid,
+                              int32_t refine, double *r_star, int32_t *at_lo, void *stream);
+
 /* (n, 8) row-major <-> 8 planes of ld.  Replaces nothing in the reference
  * (its arrays stay (n, 8)); the boundary transpose of the SoA design. */
 int32_t bh_aos_to_soa_f64(const double *aos, double *soa, int64_t ld, int64_t n,

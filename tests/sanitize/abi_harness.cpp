@@ -324,6 +324,18 @@ std::vector<Case> build_cases() {
     return bh_trace_adjoint_trajectory_f64(&p, 4, 20.0, d, 2, i64, i64, i64, i32, i64, d, d, d, d,
                                            d, d, nullptr, nullptr);
   }, "mode must be");
+  auto radial = [=](double mass, int64_t n, int32_t grid, int32_t refine, const double *spin) {
+    return bh_radial_minimum_f64(mass, spin, d, d, d, n, grid, refine, d, i32, nullptr);
+  };
+  add("radial_minimum: mass 0", [=] { return radial(0.0, 4, 256, 60, d); },
+      "mass must be positive");
+  add("radial_minimum: mass inf", [=] { return radial(__builtin_inf(), 4, 256, 60, d); },
+      "mass must be positive");
+  add("radial_minimum: n -1", [=] { return radial(1.0, -1, 256, 60, d); }, "bad n");
+  add("radial_minimum: grid 1", [=] { return radial(1.0, 4, 1, 60, d); }, "2 <= grid");
+  add("radial_minimum: refine -1", [=] { return radial(1.0, 4, 256, -1, d); }, "0 <= refine");
+  add("radial_minimum: spin NULL", [=] { return radial(1.0, 4, 256, 60, nullptr); },
+      "NULL buffer");
   add("rhs_vjp: spin NaN", [=] {
     return bh_rhs_vjp_f64(1.0, __builtin_nan(""), d, d, 4, d, d, nullptr);
   }, "Kerr bound");
