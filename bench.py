@@ -35,6 +35,7 @@ and medians.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -837,6 +838,13 @@ def run_orbits(args) -> dict:
     a = torch.tensor(0.9, dtype=torch.float64, device="cuda", requires_grad=True)
     w = torch.randn((sg.shape[0], 3), dtype=torch.float64, device="cuda",
                     generator=torch.Generator(device="cuda").manual_seed(3))
+    # warm-up on a few orbits over a short proper time: loads the kernels
+    # the timed pass uses (first launches of a module cost milliseconds)
+    sw = s0[:256].clone().requires_grad_(True)
+    Yw, _ = adjoint.trace_terminal(sw, a, dataclasses.replace(cfg, lambda_end=50.0))
+    (Yw[:, 1:4] * w[:256]).sum().backward()
+    a.grad = None
+    torch.cuda.synchronize()
     e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
     e[0].record(cur)
     Y, st_ad = adjoint.trace_terminal(sg, a, cfg)

@@ -1,6 +1,7 @@
 """Summaries of rocprofv3 result databases (SQLite, rocpd schema).
 
     python -m blackhorizon_amd.profiling stats <results.db> [--top N]
+    python -m blackhorizon_amd.profiling dispatches <results.db> [--match trace]
     python -m blackhorizon_amd.profiling pmc <results.db> [--match trace]
     python -m blackhorizon_amd.profiling traffic <fetch.db> --write-db <write.db> --match trace
 
@@ -11,7 +12,8 @@ copy ``aos_to_soa_kernel`` in the same run: that kernel reads and writes
 exactly 64 bytes per thread of its grid (n a multiple of 256).
 
 ``stats`` prints per-kernel dispatch counts and durations in ms (the
-``--kernel-trace --stats`` view).  ``pmc`` prints, per dispatch of the
+``--kernel-trace --stats`` view); ``dispatches`` every dispatch of the
+matching kernels in launch order, with its grid and duration.  ``pmc`` prints, per dispatch of the
 matching kernels, the collected counters and, for FP64 VALU counters, the
 counter-based FLOP count 64 x (ADD + MUL + TRANS + 2 FMA) (wave-level
 instruction counts times the wavefront width: inactive lanes included).
@@ -30,6 +32,15 @@ def kernel_stats(db: str, top: int = 12) -> list[tuple]:
     rows = c.execute("select name, total_calls, total_duration, average, percentage "
                      "from top_kernels order by total_duration desc").fetchall()
     return rows[:top]
+
+
+def dispatches(db: str, match: str = "") -> list[tuple]:
+    """(dispatch id, kernel, grid, workgroup, duration ms) per dispatch of
+    the matching kernels, in launch order."""
+    c = sqlite3.connect(db)
+    rows = c.execute("select dispatch_id, name, grid_x, workgroup_x, duration from kernels "
+                     "order by start").fetchall()
+    return [(d, n, g, w, dur * 1e-6) for d, n, g, w, dur in rows if match in n]
 
 
 def pmc_by_dispatch(db: str, match: str = "") -> list[dict]:
@@ -90,7 +101,7 @@ def traffic(fetch_db: str, write_db: str, match: str) -> dict:
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("what", choices=["stats", "pmc", "traffic"])
+    ap.add_argument("what", choices=["stats", "dispatches", "pmc", "traffic"])
     ap.add_argument("db")
     ap.add_argument("--top", type=int, default=12)
     ap.add_argument("--match", default="")
@@ -104,6 +115,12 @@ def main(argv=None) -> int:
             print(f"{d['kernel'][:70]} grid={d['grid']} read={d['read_bytes'] / 1e6:.3f} MB "
                   f"write={d['write_bytes'] / 1e6:.3f} MB")
         print(f"mean HBM bytes per dispatch: {t['mean_bytes_per_dispatch']:.6g}")
+        return 0
+    if a.what == "dispatches":
+        print(f"{'dispatch':>8} {'kernel':<80} {'grid':>9} {'wg':>4} {'ms':>10}")
+        for disp, name, grid, wg, ms in dispatches(a.db, a.match):
+            short = name if len(name) <= 80 else name[:77] + "..."
+            print(f"{disp:>8} {short:<80} {grid:>9} {wg:>4} {ms:>10.3f}")
         return 0
     if a.what == "stats":
         print(f"{'kernel':<90} {'calls':>6} {'total_ms':>10} {'avg_ms':>9} {'pct':>6}")
