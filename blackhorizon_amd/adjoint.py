@@ -25,6 +25,7 @@ initial states (camera.py:48-113, geodesics.py:109-170).
 
 from __future__ import annotations
 
+import dataclasses
 import math
 from dataclasses import dataclass
 
@@ -40,6 +41,12 @@ CKPT_BUDGET_BYTES = 16 << 30  # checkpoint memory reserved from max_steps alone
 TRAJ_VALUES = 9               # per stored step: start state (8) and step size
 TRAJ_PAGE_STEPS = 32          # accepted steps per page of the store (BH_TRAJ_PAGE_STEPS)
 TRAJ_BUDGET_BYTES = 64 << 30  # largest trajectory store chosen automatically
+
+# Pages of the trajectory store the last reversed recording of each
+# configuration (ray count and trace settings other than the spin) used: a
+# fit traces the same camera again and again, so the next pool is sized
+# from this rather than from the (generous) expected step counts.
+_POOL_HINTS: dict = {}
 
 
 @dataclass
@@ -102,8 +109,15 @@ class Recorded:
         self.pool_pages = 0
         pool_pages = None
         est = self._expected_steps(st, state0, cfg) if n else None
+        self._hint_key = (n, str(dev), cfg.escape_radius, cfg.max_steps, cfg.rtol, cfg.atol,
+                          cfg.initial_step, cfg.max_step, cfg.min_step, cfg.capture_margin,
+                          cfg.mass, resolve_mode(cfg.mode, cfg.rtol), cfg.lambda_end)
         if cfg.trajectory is not False:
             pages = self._pages(est, n)
+            hint = _POOL_HINTS.get(self._hint_key)
+            if hint is not None:
+                # a quarter more than last time, and a page in 16 rays
+                pages = min(pages, hint + hint // 4 + n // 16 + 64)
             if cfg.trajectory or self._store_fits(pages, dev):
                 pool_pages = pages
         if pool_pages is not None and not self._store_fits(pool_pages, dev):
@@ -188,26 +202,30 @@ class Recorded:
 
     @staticmethod
     def _expected_steps(st, state0, cfg):
-        """Per-ray expected accepted steps (batch.expected_steps), at most
-        max_steps."""
+        """Expected accepted steps (batch.expected_steps), at most max_steps:
+        one value for rays traced to escape, one per ray (host array) for
+        rays integrated to an affine end."""
         import numpy as np
 
         from .batch import TraceJob, expected_steps
 
         est = expected_steps(TraceJob(st, state0, cfg.escape_radius, rtol=cfg.rtol,
                                       max_step=cfg.max_step, lambda_end=cfg.lambda_end))
-        est = np.broadcast_to(np.asarray(est, dtype=np.float64), (state0.shape[0],))
-        return np.minimum(est, max(cfg.max_steps, 0))
+        cap = max(cfg.max_steps, 0)
+        return np.minimum(est, cap) if np.ndim(est) else min(float(est), float(cap))
 
     @staticmethod
     def _pages(steps, n):
-        """Pages holding `steps` accepted steps per ray (at least one in all)."""
+        """Pages holding `steps` accepted steps per ray (a tensor or array
+        per ray, or one number for all; at least one page in all)."""
         if not n:
             return 1
         if isinstance(steps, torch.Tensor):
             return max(int(((steps + TRAJ_PAGE_STEPS - 1) // TRAJ_PAGE_STEPS).sum().item()), 1)
         import numpy as np
 
+        if not np.ndim(steps):
+            return max(n * math.ceil(steps / TRAJ_PAGE_STEPS), 1)
         return max(int(np.ceil(np.asarray(steps) / TRAJ_PAGE_STEPS).sum()), 1)
 
     @staticmethod
@@ -266,14 +284,18 @@ class Recorded:
         n = self.n
         ld = max(n, 1)
         dev = self.term.device
-        if self.last_page is not None and int(self.workspace[1].item()) > self.pool_pages:
-            # the estimated pool fell short: size it from a plain trace's
-            # exact step counts and record again (same results)
-            st, state0 = self._redo
-            pre = self._pre_pass(st, state0, self.cfg)
-            order = torch.argsort(pre.attempts, descending=True) if self.cfg.sort_forward \
-                else None
-            self._record_store(lib, self._pages(pre.steps, n), order, _lib.stream_handle())
+        if self.last_page is not None:
+            used = int(self.workspace[1].item())
+            if used > self.pool_pages:
+                # the estimated pool fell short: size it from a plain
+                # trace's exact step counts and record again (same results)
+                st, state0 = self._redo
+                pre = self._pre_pass(st, state0, self.cfg)
+                order = torch.argsort(pre.attempts, descending=True) if self.cfg.sort_forward \
+                    else None
+                used = self._pages(pre.steps, n)
+                self._record_store(lib, used, order, _lib.stream_handle())
+            _POOL_HINTS[self._hint_key] = used
         if int((self.event < 0).sum().item()) > 0:
             raise RuntimeError("a ray exceeded its checkpoint capacity; raise cap")
         gterm = torch.zeros((8, ld), dtype=torch.float64, device=dev)
@@ -306,6 +328,11 @@ class Recorded:
         return gy0[:, :n].t().contiguous(), gspin[:n]
 
 
+# A forward that no backward will read: checkpoints only at the start (the
+# terminal points and statuses are the recording's; the store is skipped).
+_FORWARD_ONLY = dict(trajectory=False, K=1 << 20, cap=None)
+
+
 class _TraceTerminal(torch.autograd.Function):
     @staticmethod
     def forward(ctx, state0, spin, cfg):
@@ -326,6 +353,8 @@ def trace_terminal(state0: torch.Tensor, spin: torch.Tensor, cfg: TraceSettings)
     """Differentiable trace: (terminal points (n, 8), status (n,))."""
     if not isinstance(spin, torch.Tensor):
         spin = torch.tensor(float(spin), dtype=torch.float64, device=state0.device)
+    if not (torch.is_grad_enabled() and (state0.requires_grad or spin.requires_grad)):
+        cfg = dataclasses.replace(cfg, **_FORWARD_ONLY)
     return _TraceTerminal.apply(state0, spin, cfg)
 
 

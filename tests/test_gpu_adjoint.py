@@ -530,6 +530,39 @@ def test_trajectory_store_on_quads_equals_one_lane(hip):
                                       _bits(b.stored_steps(k)), err_msg=f"ray {k}")
 
 
+def test_trajectory_pool_sized_from_the_last_backward(hip, monkeypatch):
+    """A reversed recording leaves the pages it used as the next pool size
+    of the same configuration (a quarter more, plus a page per 16 rays and
+    64): the second recording's pool is that much, far below the expected
+    step counts' estimate, and its results and gradients are the first's.
+    A trace that nothing differentiates records no store at all."""
+    from blackhorizon_amd import adjoint
+
+    monkeypatch.setattr(adjoint, "_POOL_HINTS", {})
+    s0 = adjoint.camera_state(SPIN, INC, DIST, FOV, 64, 48).detach()
+    n = s0.shape[0]
+    cfg = _settings("fast", trajectory=True)
+    gY = torch.from_numpy(np.random.default_rng(8).normal(size=(n, 8))).cuda()
+    a = adjoint.Recorded(s0, SPIN, cfg)
+    ga, sa = a.backward(gY)
+    used = int(a.workspace[1])
+    assert adjoint._POOL_HINTS[a._hint_key] == used
+    b = adjoint.Recorded(s0, SPIN, cfg)
+    assert b.pool_pages == used + used // 4 + n // 16 + 64 < a.pool_pages
+    gb, sb = b.backward(gY)
+    np.testing.assert_array_equal(_bits(ga), _bits(gb))
+    np.testing.assert_array_equal(_bits(sa), _bits(sb))
+    stores = []
+    real = adjoint.Recorded._record_store
+    monkeypatch.setattr(adjoint.Recorded, "_record_store",
+                        lambda self, *args: stores.append(1) or real(self, *args))
+    with torch.no_grad():
+        Y, st = adjoint.trace_terminal(s0, SPIN, cfg)
+    assert not stores
+    np.testing.assert_array_equal(_bits(Y), _bits(a.terminal()))
+    np.testing.assert_array_equal(st.cpu().numpy(), a.status.cpu().numpy())
+
+
 def test_trajectory_pool_too_small(hip, monkeypatch):
     """A pool that runs out: the rays that find it empty stop storing and
     get event -1, nothing lands past the pool, and the pool counter reports
@@ -570,6 +603,7 @@ def test_trajectory_pool_too_small(hip, monkeypatch):
     assert (link[short:] == -7).all()
     # Recorded with an estimate that falls short: the forward's results are
     # right; the backward sizes the pool exactly and records again
+    monkeypatch.setattr(adjoint, "_POOL_HINTS", {})
     monkeypatch.setattr(adjoint.Recorded, "_expected_steps",
                         staticmethod(lambda st, s, cfg: np.ones(s.shape[0])))
     b = adjoint.Recorded(s0, SPIN, _settings("precise", trajectory=True))
@@ -580,6 +614,7 @@ def test_trajectory_pool_too_small(hip, monkeypatch):
     ga, _ = ref.backward(gY)
     gb, _ = b.backward(gY)
     assert b.pool_pages == need == int(b.workspace[1])
+    assert adjoint._POOL_HINTS[b._hint_key] == need
     np.testing.assert_array_equal(_bits(ga), _bits(gb))
     np.testing.assert_array_equal(_bits(ref.term[:, :n]), _bits(b.term[:, :n]))
     np.testing.assert_array_equal(_bits(ref.event), _bits(b.event))
