@@ -93,6 +93,7 @@ struct RecordIO {
 // accepted step size.
 constexpr int TRAJ_VALUES = 9;
 constexpr int TRAJ_PAGE_STEPS = BH_TRAJ_PAGE_STEPS;
+constexpr int TRAJ_RUN_MAX = BH_TRAJ_RUN_MAX;
 static_assert((TRAJ_PAGE_STEPS & (TRAJ_PAGE_STEPS - 1)) == 0, "page size: a power of two");
 
 struct AdjointIO {

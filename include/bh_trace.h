@@ -197,22 +197,28 @@ int32_t bh_trace_record_f64(const bh_trace_params *p, const double *state_in_soa
                             int64_t *ev_step, double *theta, int32_t *event, uint64_t *workspace,
                             void *stream);
 
-/* Accepted steps per page of a trajectory store. */
+/* Accepted steps per page of a trajectory store, and the longest run of
+ * consecutive pages a ray takes from the pool at once (its runs are 1, 2,
+ * 4, 8, ... pages, then BH_TRAJ_RUN_MAX each). */
 #define BH_TRAJ_PAGE_STEPS 32
+#define BH_TRAJ_RUN_MAX 16
 
 /* bh_trace_record_f64 storing every accepted step instead of checkpoints,
  * for bh_trace_adjoint_trajectory_f64 (no replay in the reverse pass).  The
  * store is a pool of pool_pages pages of BH_TRAJ_PAGE_STEPS steps (pool:
- * pool_pages * BH_TRAJ_PAGE_STEPS * 9 device doubles); a ray takes a page
- * from the pool, through an atomic counter, when its accepted step s is a
- * multiple of BH_TRAJ_PAGE_STEPS, and writes step s (start state, then the
- * accepted step size) at pool[(page * BH_TRAJ_PAGE_STEPS + s mod
- * BH_TRAJ_PAGE_STEPS) * 9 + 0..8].  page_link [pool_pages]: each taken
+ * pool_pages * BH_TRAJ_PAGE_STEPS * 9 device doubles); a ray opens a page
+ * when its accepted step s is a multiple of BH_TRAJ_PAGE_STEPS -- the next
+ * page of the run of consecutive pages it last took from the pool through
+ * an atomic counter, or, with that run used up, the first of a new run
+ * (runs of 1, 2, 4, ... pages, at most BH_TRAJ_RUN_MAX) -- and writes step
+ * s (start state, then the accepted step size) at pool[(page *
+ * BH_TRAJ_PAGE_STEPS + s mod BH_TRAJ_PAGE_STEPS) * 9 + 0..8].  page_link [pool_pages]: each taken
  * page's predecessor of the same ray (-1 for its first page); last_page [n]:
  * each ray's last page (-1 without steps).  workspace: device uint64[2]
  * (the quad form's ray queue, the pool counter); after the call
- * workspace[1] holds the pages requested: above pool_pages, the rays that
- * found the pool empty stopped storing and have event -1.  Every output
+ * workspace[1] holds the pages requested (runs included in full): above
+ * pool_pages, the rays whose run did not fit stopped storing and have
+ * event -1.  Every output
  * other than the store is bh_trace_record_f64's, bit for bit. */
 int32_t bh_trace_record_trajectory_f64(
     const bh_trace_params *p, const double *state_in_soa, int64_t ld_in, double *state_out_soa,

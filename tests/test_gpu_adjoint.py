@@ -460,6 +460,20 @@ def test_capture_margin_search_kernel_matches_torch(hip):
         assert abs(float(gd) - float(gh)) <= 1e-7 * abs(float(gh))
 
 
+def _run_pages(steps):
+    """Pages a recording takes for these per-ray step counts: each ray
+    takes runs of 1, 2, 4, ... 16 consecutive 32-step pages until they hold
+    its steps (counted one ray at a time)."""
+    total = 0
+    for s in np.asarray(steps).tolist():
+        need, got, run = -(-s // 32), 0, 1
+        while got < need:
+            got += run
+            run = min(2 * run, 16)
+        total += got
+    return total
+
+
 def _bits(t):
     t = t.detach().contiguous().cpu().numpy()
     return t.view(np.int64) if t.dtype == np.float64 else t
@@ -476,7 +490,8 @@ def test_trajectory_store_gives_the_same_bits(hip, mode, width, height):
     the initial states and the spin.  Camera rays escape and are captured
     (both event kinds); 3,072 precise rays record on quads, 70,000 on one
     lane per ray.  The pool counter equals the pages the rays' step counts
-    need, each ray's page chain holds its steps in order (its first stored
+    need in runs of 1, 2, 4, ... 16 pages, each ray's page chain holds its
+    steps in order (its first stored
     state is its initial state), and every checkpoint equals the stored
     start state of its step."""
     from blackhorizon_amd import adjoint
@@ -495,8 +510,7 @@ def test_trajectory_store_gives_the_same_bits(hip, mode, width, height):
     ev = b.event.cpu().numpy()
     assert (ev == 1).any() and (ev == 2).any() and (ev >= 0).all()
     steps = b.steps.cpu().numpy()
-    P = adjoint.TRAJ_PAGE_STEPS
-    assert int(b.workspace[1]) == int(np.sum(-(-steps // P))) <= b.pool_pages
+    assert int(b.workspace[1]) == _run_pages(steps) <= b.pool_pages
     ck, nck = a.ckpt.cpu().numpy(), a.n_ckpt.cpu().numpy()
     K = a.cfg.K
     for k in range(0, n, 97):
@@ -533,8 +547,9 @@ def test_trajectory_store_on_quads_equals_one_lane(hip):
 def test_trajectory_pool_sized_from_the_last_backward(hip, monkeypatch):
     """A reversed recording leaves the pages it used as the next pool size
     of the same configuration (a quarter more, plus a page per 16 rays and
-    64): the second recording's pool is that much, far below the expected
-    step counts' estimate, and its results and gradients are the first's.
+    64): the second recording's pool is that much (or the expected step
+    counts' estimate, when smaller), and its results and gradients are the
+    first's.
     A trace that nothing differentiates records no store at all."""
     from blackhorizon_amd import adjoint
 
@@ -548,7 +563,8 @@ def test_trajectory_pool_sized_from_the_last_backward(hip, monkeypatch):
     used = int(a.workspace[1])
     assert adjoint._POOL_HINTS[a._hint_key] == used
     b = adjoint.Recorded(s0, SPIN, cfg)
-    assert b.pool_pages == used + used // 4 + n // 16 + 64 < a.pool_pages
+    estimate = adjoint.Recorded._pages(adjoint.Recorded._expected_steps(None, s0, cfg), n)
+    assert b.pool_pages == min(estimate, used + used // 4 + n // 16 + 64)
     gb, sb = b.backward(gY)
     np.testing.assert_array_equal(_bits(ga), _bits(gb))
     np.testing.assert_array_equal(_bits(sa), _bits(sb))
@@ -575,7 +591,7 @@ def test_trajectory_pool_too_small(hip, monkeypatch):
     s0 = adjoint.camera_state(SPIN, INC, DIST, FOV, 16, 12).detach()
     n = s0.shape[0]
     ref = adjoint.Recorded(s0, SPIN, _settings("precise", trajectory=False))
-    need = int(np.sum(-(-ref.steps.cpu().numpy() // adjoint.TRAJ_PAGE_STEPS)))
+    need = _run_pages(ref.steps.cpu().numpy())
     lib = _lib.require_device()
     short = need // 2
     guard = 4  # pages past the pool, filled with a sentinel
