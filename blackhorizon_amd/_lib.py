@@ -28,7 +28,7 @@ MODE_PRECISE = 1
 DENSE_DONE = 0
 DENSE_CHUNK_FULL = 1
 DENSE_UNDERFLOW = 2
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 
 class TraceParams(ctypes.Structure):

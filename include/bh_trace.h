@@ -30,8 +30,9 @@
 extern "C" {
 #endif
 
-#define BH_ABI_VERSION 3  /* 2: integrate_dense work is 24 ld + 8192 doubles; static-scratch adjoint;
-                             3: bh_tune_params.lanes_per_ray (quad tracer) */
+#define BH_ABI_VERSION 4  /* 2: integrate_dense work is 24 ld + 8192 doubles; static-scratch adjoint;
+                             3: bh_tune_params.lanes_per_ray (quad tracer);
+                             4: trajectory-store record / reverse pass, radial minimum */
 
 #define BH_OK 0
 #define BH_EINVAL -1   /* invalid argument (maps to ValueError) */
