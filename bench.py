@@ -193,9 +193,13 @@ def lane_efficiency(plan, rays_per_lane):
     return float(a.sum().item()) / float((64 * w.max(dim=1).values).sum().item())
 
 
-def pick_tune(args, prepared, kind_key):
+def pick_tune(args, prepared, kind_key, workload=None):
     """--tune: "auto" runs the autotuner on the first plan (its table goes
-    into the JSON line), "default" is (256, 1, 1), "B,R,M" is explicit."""
+    into the JSON line), then times its three fastest shapes on every plan
+    of the workload (``workload``, default ``prepared``) and keeps the one
+    with the least total (shapes within noise on one frame can differ by
+    several per cent over the sweep);
+    "default" is (256, 1, 1), "B,R,M" is explicit."""
     from blackhorizon_amd import autotune, tracer
 
     if args.tune == "default":
@@ -206,11 +210,22 @@ def pick_tune(args, prepared, kind_key):
     plan = prepared[0][1]
     res = autotune.autotune(plan, workload_key=kind_key, cache=None, reps=2)
     print(f"[bench] autotune: {res.best} ({res.best_ms:.3f} ms)", file=sys.stderr, flush=True)
+    ranked = sorted(res.table, key=lambda e: e[1])
+    best = res.best
+    totals = []
+    workload = workload or prepared
+    if len(workload) > 1:
+        for t, _ in ranked[:3]:
+            tune = tracer.Tune(**t)
+            total = sum(autotune.time_candidates(p, [tune], reps=2)[0][1] for _, p in workload)
+            totals.append([t, round(total, 4)])
+        best = tracer.Tune(**min(totals, key=lambda e: e[1])[0])
+        print(f"[bench] over the workload: {best}", file=sys.stderr, flush=True)
     info = dict(autotuned_on=kind_key, best_ms=res.best_ms,
                 candidates=len(res.table),
-                fastest5=sorted(([t, round(ms, 4)] for t, ms in res.table),
-                                key=lambda e: e[1])[:5])
-    return res.best, info
+                fastest5=[[t, round(ms, 4)] for t, ms in ranked[:5]],
+                workload_ms_of_fastest3=totals)
+    return best, info
 
 
 def frame_io(job, mode: str, tune, reps: int = 5) -> dict:
@@ -940,7 +955,7 @@ def run_dense(args) -> dict:
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="hd",
                     choices=["hd", "hd09", "small", "uneven", "shadowfit", "orbits", "rk4",
@@ -1038,7 +1053,8 @@ def main(argv=None) -> int:
     prepared = prepare_gpu(jobs, args.mode, args.order)
     # tune on the a = 0.9, 17 deg frame when present (else the first)
     tune_plans = [jp for jp in prepared if jp[0].spin == 0.9] or prepared
-    tune, tune_info = pick_tune(args, tune_plans, f"{args.workload}-a0.9-{args.order}")
+    tune, tune_info = pick_tune(args, tune_plans, f"{args.workload}-a0.9-{args.order}",
+                                workload=prepared)
     cur = torch.cuda.current_stream()
 
     for w in range(args.warmup):
