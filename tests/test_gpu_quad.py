@@ -4,6 +4,7 @@ one-lane batched tracer gives it -- states compared as raw 64-bit patterns,
 so signed zeros count -- on camera jobs of the uneven mix, on the failure
 and cap edge cases, and on the whole 65,536-orbit timelike workload."""
 
+import math
 import time
 
 import numpy as np
@@ -144,3 +145,25 @@ def test_quad_orbit_workload_full_size(hip):
     print(f"orbit workload: one lane per orbit {times[1] * 1e3:.1f} ms, "
           f"a quad per orbit {times[4] * 1e3:.1f} ms")
     assert times[4] < times[1]
+
+
+def test_integrate_rays_takes_the_quad_path_with_the_same_bits(hip):
+    """integrate_rays of at most one wavefront per SIMD in the precise build
+    runs on the quad tracer: 3,000 orbits to tau = 300 give what the plain
+    one-lane launch (trace_rays with the same lambda_end, and
+    integrate_rays with an explicit launch shape) gives, bit for bit."""
+    import torch
+
+    from blackhorizon_amd import geodesics, kerr, tracer
+
+    pos, vel = workloads.near_circular_orbits(3000, seed=21)
+    st = kerr.KerrSpacetime(1.0, 0.9)
+    p = torch.from_numpy(pos).cuda()
+    s0 = geodesics.build_state(p, geodesics.timelike_momentum(st, p, torch.from_numpy(vel).cuda()))
+    kw = dict(rtol=1e-10, atol=1e-12, initial_step=0.05, min_step=1e-14, max_steps=10_000_000)
+    quad = tracer.integrate_rays(st, s0, 300.0, **kw)
+    plain = tracer.trace_rays(st, s0, math.inf, mode="precise", lambda_end=300.0, **kw)
+    tuned = tracer.integrate_rays(st, s0, 300.0, tune=tracer.Tune(), **kw)
+    for other in (plain, tuned):
+        _assert_bits(quad, other, "integrate_rays")
+    assert (_np(quad.status) == 6).mean() > 0.95
