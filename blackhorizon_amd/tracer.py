@@ -214,13 +214,41 @@ def trace_rays(
     permutation) sets which rays share a wavefront -- e.g.
     ``PinholeCamera.tile_order()`` -- without changing any result.
     ``lambda_end`` (affine parameter) ends rays that reach it with status
-    TERMINATED (see integrate_rays).
+    TERMINATED (see integrate_rays).  Without ``tune`` and ``ray_order``, a
+    precise-build trace of at most one wavefront per compute unit runs on
+    the quad tracer (a DPP quad of lanes per ray, the same results bit for
+    bit), which shortens the long rays such a launch waits for.
     """
+    if tune is None and ray_order is None and _quads_pay(state0, mode, rtol, 1):
+        return _trace_on_quads(spacetime, state0, escape_radius, max_steps, rtol, atol,
+                               initial_step, max_step, min_step, capture_margin, lambda_end)
     plan = TracePlan(spacetime, state0, escape_radius, max_steps, rtol, atol, initial_step,
                      max_step, min_step, capture_margin, mode=mode, tune=tune,
                      ray_order=ray_order, lambda_end=lambda_end)
     plan.launch()
     return plan.result()
+
+
+def _quads_pay(state0, mode: str, rtol: float, waves_per_cu: int) -> bool:
+    """A precise-build launch of at most ``waves_per_cu`` one-lane
+    wavefronts per compute unit: its time is its longest rays', which the
+    quad tracer shortens (one lane per ray wins once the launch fills the
+    chip: camera rays at 65,536 rays, where the tail no longer sets it)."""
+    if resolve_mode(mode, rtol) != "precise":
+        return False
+    shape = getattr(state0, "shape", ())
+    n = int(shape[0]) if len(shape) == 2 and shape[1] == 8 else 0
+    cus = torch.cuda.get_device_properties(_dev.device()).multi_processor_count
+    return 0 < n and -(-n // 64) <= waves_per_cu * cus
+
+
+def _trace_on_quads(spacetime, state0, escape_radius, max_steps, rtol, atol, initial_step,
+                    max_step, min_step, capture_margin, lambda_end) -> TraceResult:
+    from .batch import TraceJob, trace_batch
+
+    job = TraceJob(spacetime, state0, escape_radius, max_steps, rtol, atol, initial_step,
+                   max_step, min_step, capture_margin, lambda_end)
+    return trace_batch([job], mode="precise", tune=Tune(lanes_per_ray=4))[0]
 
 
 def integrate_rays(
@@ -258,16 +286,10 @@ def integrate_rays(
     208 ms against 289 ms)."""
     if not lambda_total > 0.0:
         raise ValueError("lambda_total must be positive")
-    if tune is None and resolve_mode(mode, rtol) == "precise":
-        n = int(state0.shape[0])
-        cus = torch.cuda.get_device_properties(_dev.device()).multi_processor_count
-        if 0 < n and -(-n // 64) <= 4 * cus:
-            from .batch import TraceJob, trace_batch
-
-            job = TraceJob(spacetime, state0, escape_radius, max_steps, rtol, atol,
-                           min(initial_step, max_step), max_step, min_step, capture_margin,
-                           float(lambda_total))
-            return trace_batch([job], mode="precise", tune=Tune(lanes_per_ray=4))[0]
+    if tune is None and _quads_pay(state0, mode, rtol, 4):
+        return _trace_on_quads(spacetime, state0, escape_radius, max_steps, rtol, atol,
+                               min(initial_step, max_step), max_step, min_step, capture_margin,
+                               float(lambda_total))
     return trace_rays(spacetime, state0, escape_radius, max_steps=max_steps, rtol=rtol,
                       atol=atol, initial_step=min(initial_step, max_step), max_step=max_step,
                       min_step=min_step, capture_margin=capture_margin, mode=mode, tune=tune,

@@ -326,17 +326,26 @@ def test_trace_edge_cases(hip, fixture, case, mode):
     (FAILED; the fast build's NaN-dropping max in the error norm must still
     end them FAILED), max_steps 0 / negative / 1 / 7 and the 4 max_steps
     attempt cap (MAX_STEPS), h below min_step (FAILED), tiny and huge steps,
-    a ray starting inside the horizon."""
+    a ray starting inside the horizon.  Precise-build traces this small run
+    on the quad tracer by default; with an explicit launch shape they run
+    one lane per ray: both are checked."""
+    from blackhorizon_amd.tracer import resolve_mode
+
     g = golden(fixture)
     kw = _kw(g, case)
-    res = hip.trace_rays(hip.KerrSpacetime(spin=0.9), g[case + "_state0"], mode=mode, **kw)
-    assert res.iterations == int(g[case + "_iterations"])
+    shapes = [None]
+    if resolve_mode(mode, kw.get("rtol", 1e-9)) == "precise":
+        shapes.append(hip.Tune())
     spread = _spread((fixture, case), 0.9, g[case + "_state0"], g[case + "_states"], kw)
-    assert_trace_parity(res.states, res.status, res.steps, g[case + "_states"],
-                        g[case + "_status"], g[case + "_steps"], label=f"{fixture}:{case}:{mode}",
-                        spread=spread)
-    assert np.all(res.attempts >= res.steps)
-    assert int(np.max(res.attempts, initial=0)) == res.iterations
+    for tune in shapes:
+        res = hip.trace_rays(hip.KerrSpacetime(spin=0.9), g[case + "_state0"], mode=mode,
+                             tune=tune, **kw)
+        label = f"{fixture}:{case}:{mode}:{'one-lane' if tune else 'default'}"
+        assert res.iterations == int(g[case + "_iterations"]), label
+        assert_trace_parity(res.states, res.status, res.steps, g[case + "_states"],
+                            g[case + "_status"], g[case + "_steps"], label=label, spread=spread)
+        assert np.all(res.attempts >= res.steps)
+        assert int(np.max(res.attempts, initial=0)) == res.iterations
 
 
 @pytest.mark.parametrize("case", [c for c in EDGE_CASES if c != "loose"])
