@@ -46,8 +46,17 @@ TRAJ_BUDGET_BYTES = 96 << 30  # largest trajectory store chosen automatically
 # Pages of the trajectory store the last reversed recording of each
 # configuration (ray count and trace settings other than the spin) used: a
 # fit traces the same camera again and again, so the next pool is sized
-# from this rather than from the (generous) expected step counts.
+# from this rather than from the (generous) expected step counts.  At most
+# _POOL_HINTS_MAX configurations are kept; the least recently stored go first.
 _POOL_HINTS: dict = {}
+_POOL_HINTS_MAX = 64
+
+
+def _remember_pages(key, pages: int) -> None:
+    _POOL_HINTS.pop(key, None)
+    _POOL_HINTS[key] = int(pages)
+    while len(_POOL_HINTS) > _POOL_HINTS_MAX:
+        del _POOL_HINTS[next(iter(_POOL_HINTS))]
 
 
 @dataclass
@@ -327,11 +336,11 @@ class Recorded:
                     else None
                 used = self._pages(pre.steps, n)
                 self._record_store(lib, used, order, _lib.stream_handle())
-            _POOL_HINTS[self._hint_key] = used
+            _remember_pages(self._hint_key, used)
         elif self.cfg.trajectory is None and n:
             # checkpoints this time (the estimate did not fit): the step
             # counts size the next recording's store
-            _POOL_HINTS[self._hint_key] = self._pages(self.steps, n)
+            _remember_pages(self._hint_key, self._pages(self.steps, n))
         if int((self.event < 0).sum().item()) > 0:
             raise RuntimeError("a ray exceeded its checkpoint capacity; raise cap")
         gterm = torch.zeros((8, ld), dtype=torch.float64, device=dev)

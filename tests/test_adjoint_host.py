@@ -77,3 +77,15 @@ def test_store_budget_rule(monkeypatch):
     limit = min(adjoint.TRAJ_BUDGET_BYTES, Props.total_memory // 3)
     assert Recorded._store_fits(limit // per_page, "cuda")
     assert not Recorded._store_fits(limit // per_page + 1, "cuda")
+
+
+def test_pool_hints_keep_the_most_recent(monkeypatch):
+    monkeypatch.setattr(adjoint, "_POOL_HINTS", {})
+    for k in range(adjoint._POOL_HINTS_MAX + 10):
+        adjoint._remember_pages(("cfg", k), k)
+    adjoint._remember_pages(("cfg", 10), 99)  # stored again: now the newest
+    adjoint._remember_pages(("cfg", "new"), 1)
+    assert len(adjoint._POOL_HINTS) == adjoint._POOL_HINTS_MAX
+    assert ("cfg", 10) in adjoint._POOL_HINTS and adjoint._POOL_HINTS[("cfg", 10)] == 99
+    assert ("cfg", 11) not in adjoint._POOL_HINTS  # the oldest left
+    assert ("cfg", adjoint._POOL_HINTS_MAX + 9) in adjoint._POOL_HINTS
