@@ -270,7 +270,7 @@ def test_full_size_fit_gradients(hip):
         L = _loss(p, (dirs, st_t), _settings(K=K), size=size)
         L.backward()
         grads[K] = [float(t.grad) for t in p]
-        print(f"K = {K}: loss {float(L):.12g} grads {grads[K]}")
+        print(f"K = {K}: loss {float(L.detach()):.12g} grads {grads[K]}")
     assert grads[32] == grads[8]
     _fd_check((dirs, st_t), cfg32, (1, 2), (0,), 1e-5, size=size)
 
