@@ -673,7 +673,29 @@ def run_shadowfit(args) -> dict:
         print(f"[bench] shadowfit pass {k} done", file=sys.stderr, flush=True)
     med = {k: float(np.median(v)) for k, v in times.items()}
     steps = int(res.steps.sum().item())
-    return dict(rays=size * size, ray_steps=steps, loss=float(loss.detach()),
+    profiled = {}
+    if args.torch_profile:
+        # one more fit step under the PyTorch profiler: where its device time goes
+        from torch.profiler import ProfilerActivity, profile
+
+        torch.cuda.synchronize()
+        with profile(activities=[ProfilerActivity.CPU, ProfilerActivity.CUDA]) as prof:
+            t0 = time.perf_counter()
+            y0 = adjoint.camera_state(p[0], p[1], p[2], 30.0, size, size)
+            Y, status = adjoint.trace_terminal(y0, p[0], cfg)
+            loss = adjoint.shadow_loss(Y, status, p[0], dirs_t, st_t, state0=y0)
+            g = torch.autograd.grad(loss, p)
+            torch.cuda.synchronize()
+            wall = (time.perf_counter() - t0) * 1e3
+        kern = [e for e in prof.events() if e.device_type.name == "CUDA"]
+        busy = sum(e.device_time for e in kern) / 1e3
+        tracer_ms = sum(e.device_time for e in kern if e.name.startswith(("bh::", "void bh::"))) / 1e3
+        print(prof.key_averages().table(sort_by="cuda_time_total", row_limit=25,
+                                        max_name_column_width=70), file=sys.stderr, flush=True)
+        profiled = dict(profiled_fit_step=dict(wall_ms=wall, device_busy_ms=busy,
+                                               tracer_kernel_ms=tracer_ms,
+                                               device_launches=len(kern)))
+    return dict(**profiled, rays=size * size, ray_steps=steps, loss=float(loss.detach()),
                 grad_spin=float(g[0]), grad_inclination=float(g[1]), grad_distance=float(g[2]),
                 ms_plain_forward=med["plain"], ms_recording_forward=med["record"],
                 ms_adjoint=med["adjoint"], ms_fit_step=med["fit_step"],
@@ -965,6 +987,8 @@ def main(argv=None) -> int:
     ap.add_argument("--orbits", type=int, default=65536, help="orbits: number of orbits")
     ap.add_argument("--proper-time", type=float, default=1.0e4, help="orbits: proper time")
     ap.add_argument("--fit-size", type=int, default=512, help="shadowfit: image side")
+    ap.add_argument("--torch-profile", action="store_true",
+                    help="shadowfit: one more fit step under the PyTorch profiler")
     ap.add_argument("--fit-image", type=int, default=512, help="fit: image side")
     ap.add_argument("--offline-size", type=int, default=1920,
                     help="offline: frame width (height = 5/8 of it)")
